@@ -1,0 +1,267 @@
+"""Python binding of libcgmlst_hamming.so (ctypes over the C ABI in
+include/cgmlst_hamming.h).
+
+The product is the C ABI library and the `cgmlst-hamming-gpu` command line;
+this module is a thin host-side mirror used by the tests and the benchmark.
+It mirrors the reference's host pipeline (tensorcore/rs/src/main.rs:353-465:
+dense ranks -> tau -> O/P bits + rare pairs -> GPU stripes) with the same
+argument meaning and error behaviour (failures raise CghError carrying the
+library's message, the reference's `die` text).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for the distance computation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcgmlst_hamming.so")
+CLI_PATH = os.path.join(HERE, "bin", "cgmlst-hamming-gpu")
+
+# Symbols the header declares, in declaration order.
+EXPORTS = (
+    "tch_device_count", "tch_free_vram", "tch_run",
+    "cgh_dense_ranks", "cgh_encode", "cgh_rare_csr", "cgh_encoding_free",
+    "cgh_allele_counts", "cgh_choose_tau",
+    "cgh_dataset_upload", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
+    "cgh_dataset_upper", "cgh_dataset_tile",
+)
+
+
+class CghError(RuntimeError):
+    pass
+
+
+class Encoding(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("L", C.c_int), ("tau", C.c_int),
+        ("n_pad", C.c_size_t), ("D", C.c_size_t), ("kw1", C.c_size_t), ("kw2", C.c_size_t),
+        ("O", C.POINTER(C.c_uint32)), ("P", C.POINTER(C.c_uint32)),
+        ("n_groups", C.c_size_t), ("n_members", C.c_size_t),
+        ("gstart", C.POINTER(C.c_uint32)), ("gmem", C.POINTER(C.c_uint32)),
+        ("gpairs", C.POINTER(C.c_uint64)), ("n_rare", C.c_uint64),
+    ]
+
+
+TCH_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.POINTER(C.c_uint16))
+LOG_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
+
+_lib = None
+
+
+def load():
+    """Load the built library (building is __graft_entry__.build()'s job)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CghError(f"{LIB_PATH} is not built: run __graft_entry__.build() (make -C cgmlst-hamming-gpu_amd)")
+    lib = C.CDLL(LIB_PATH)
+    P, S, I, U64 = C.c_void_p, C.c_size_t, C.c_int, C.c_uint64
+    sig = {
+        "tch_device_count": (I, []),
+        "tch_free_vram": (S, [I]),
+        "tch_run": (I, [I, I, S, P, S, P, S, P, P, S, I, TCH_CB, P, C.POINTER(C.c_double), C.POINTER(I), C.c_char_p, I]),
+        "cgh_dense_ranks": (I, [P, I, I, P, C.c_char_p, I]),
+        "cgh_encode": (I, [P, I, I, I, I, C.POINTER(Encoding), C.c_char_p, I]),
+        "cgh_rare_csr": (I, [C.POINTER(Encoding), P, P, C.c_char_p, I]),
+        "cgh_encoding_free": (None, [C.POINTER(Encoding)]),
+        "cgh_allele_counts": (S, [P, I, I, P, S]),
+        "cgh_choose_tau": (I, [P, S, I, I, I, S, LOG_CB, P]),
+        "cgh_dataset_upload": (I, [I, C.POINTER(Encoding), C.POINTER(P), C.c_char_p, I]),
+        "cgh_dataset_free": (None, [P]),
+        "cgh_upper_elems": (S, [I, I]),
+        "cgh_stripe_offset": (S, [I, I, I]),
+        "cgh_dataset_upper": (I, [P, I, P, P, I, C.c_char_p, I]),
+        "cgh_dataset_tile": (I, [P, I]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _err():
+    return C.create_string_buffer(512)
+
+
+def _check(rc, err):
+    if rc:
+        raise CghError(err.value.decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    return load().tch_device_count()
+
+
+def free_vram(device: int = 0) -> int:
+    return load().tch_free_vram(device)
+
+
+def dense_ranks(calls: np.ndarray) -> np.ndarray:
+    """u64 allele ids (n x L, 0 = missing) -> u16 dense per-locus ranks
+    (main.rs:135-165)."""
+    calls = np.ascontiguousarray(calls, dtype=np.uint64)
+    n, L = calls.shape
+    X = np.zeros((n, L), dtype=np.uint16)
+    err = _err()
+    _check(load().cgh_dense_ranks(_ptr(calls), n, L, _ptr(X), err, 512), err)
+    return X
+
+
+def allele_counts(X: np.ndarray) -> np.ndarray:
+    X = np.ascontiguousarray(X, dtype=np.uint16)
+    n, L = X.shape
+    lib = load()
+    need = lib.cgh_allele_counts(_ptr(X), n, L, None, 0)
+    cnt = np.zeros(max(need, 1), dtype=np.uint32)
+    lib.cgh_allele_counts(_ptr(X), n, L, _ptr(cnt), need)
+    return cnt[:need]
+
+
+def choose_tau(X: np.ndarray, stripe: int, free_bytes: int, log=None) -> int:
+    """Auto tau (main.rs:169-198 rule, MI355X rates); 0 if none fits."""
+    n, L = X.shape
+    cnt = allele_counts(X)
+    lines = []
+    cb = LOG_CB(lambda ctx, line: lines.append(line.decode()))
+    tau = load().cgh_choose_tau(_ptr(cnt), cnt.size, n, L, stripe, free_bytes, cb, None)
+    if log is not None:
+        for ln in lines:
+            log(ln)
+    return tau
+
+
+class Encoded:
+    """Host operands of one profile set (cgh_encode)."""
+
+    def __init__(self, X: np.ndarray, tau: int, k_align: int = 256):
+        X = np.ascontiguousarray(X, dtype=np.uint16)
+        n, L = X.shape
+        self.enc = Encoding()
+        err = _err()
+        _check(load().cgh_encode(_ptr(X), n, L, int(tau), int(k_align), C.byref(self.enc), err, 512), err)
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "enc", None) is not None:
+            _lib.cgh_encoding_free(C.byref(self.enc))
+            self.enc = None
+
+    def _arr(self, field, count, dtype):
+        p = getattr(self.enc, field)
+        if not p or count == 0:
+            return np.zeros(0, dtype=dtype)
+        return np.ctypeslib.as_array(p, shape=(count,)).copy()
+
+    @property
+    def O(self):
+        e = self.enc
+        return self._arr("O", e.n_pad * e.kw1, np.uint32).reshape(e.n_pad, e.kw1)
+
+    @property
+    def P(self):
+        e = self.enc
+        return self._arr("P", e.n_pad * e.kw2, np.uint32).reshape(e.n_pad, e.kw2)
+
+    def groups(self):
+        e = self.enc
+        return (self._arr("gstart", e.n_groups + 1, np.uint32), self._arr("gmem", e.n_members, np.uint32),
+                self._arr("gpairs", e.n_groups + 1, np.uint64))
+
+    def rare_csr(self):
+        e = self.enc
+        start = np.zeros(e.n + 1, dtype=np.uint32)
+        js = np.zeros(max(e.n_rare, 1), dtype=np.uint32)
+        err = _err()
+        _check(load().cgh_rare_csr(C.byref(self.enc), _ptr(start), _ptr(js), err, 512), err)
+        return start, js[: e.n_rare]
+
+
+def upper_elems(n: int, S: int) -> int:
+    return load().cgh_upper_elems(n, S)
+
+
+def stripe_offset(n: int, S: int, s: int) -> int:
+    return load().cgh_stripe_offset(n, S, s)
+
+
+class Dataset:
+    """Device-resident operands (cgh_dataset_upload)."""
+
+    def __init__(self, enc: Encoded, device: int = 0):
+        self.n = enc.enc.n
+        self.handle = C.c_void_p()
+        err = _err()
+        _check(load().cgh_dataset_upload(device, C.byref(enc.enc), C.byref(self.handle), err, 512), err)
+
+    def close(self):
+        if _lib is not None and self.handle:
+            _lib.cgh_dataset_free(self.handle)
+            self.handle = C.c_void_p()
+
+    __del__ = close
+
+    def tile(self, kernel: int = 0) -> int:
+        return load().cgh_dataset_tile(self.handle, kernel)
+
+    def upper(self, S: int, out_ptr: int, stream_ptr: int = 0, kernel: int = 0):
+        """Enqueue the exact upper triangle into device memory out_ptr."""
+        err = _err()
+        _check(load().cgh_dataset_upper(self.handle, S, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), kernel, err,
+                                        512), err)
+
+
+def upper_to_square(flat: np.ndarray, n: int, S: int) -> np.ndarray:
+    """Stripe-layout upper triangle (u16) -> symmetric n x n matrix."""
+    H = np.zeros((n, n), dtype=np.uint16)
+    for s, r0 in enumerate(range(0, n, S)):
+        rows, w = min(S, n - r0), n - r0
+        off = stripe_offset(n, S, s)
+        blk = flat[off: off + rows * w].reshape(rows, w)
+        H[r0:r0 + rows, r0:] = blk
+    iu = np.triu_indices(n, 1)
+    H.T[iu] = H[iu]
+    np.fill_diagonal(H, 0)
+    return H
+
+
+def tch_run(enc: Encoded, S: int = 512, device: int = 0):
+    """Run the reference-compatible tch_run on the encoding (K aligned to
+    1024) and return (H upper-triangle rows as a square matrix, gpu_secs,
+    rare_on_gpu)."""
+    e = enc.enc
+    if (e.kw1 * 32) % 1024 or (e.kw2 * 32) % 1024:
+        raise CghError("tch_run needs an encoding with k_align=1024")
+    n = e.n
+    start, js = enc.rare_csr()
+    js_buf = js if js.size else np.zeros(1, dtype=np.uint32)
+    H = np.zeros((n, n), dtype=np.uint16)
+    seen = []
+
+    def cb(ctx, r0, rows, w, stripe):
+        blk = np.ctypeslib.as_array(stripe, shape=(rows * w,)).reshape(rows, w)
+        H[r0:r0 + rows, r0:] = blk
+        seen.append((r0, rows, w))
+
+    ccb = TCH_CB(cb)
+    secs = C.c_double()
+    rare_gpu = C.c_int()
+    err = _err()
+    rc = load().tch_run(device, n, e.n_pad, C.cast(e.O, C.c_void_p), e.kw1 * 32, C.cast(e.P, C.c_void_p), e.kw2 * 32,
+                        _ptr(start), _ptr(js_buf), int(e.n_rare), S, ccb, None, C.byref(secs), C.byref(rare_gpu),
+                        err, 512)
+    _check(rc, err)
+    iu = np.triu_indices(n, 1)
+    H.T[iu] = H[iu]
+    np.fill_diagonal(H, 0)
+    return H, secs.value, bool(rare_gpu.value), seen

@@ -1,0 +1,623 @@
+// cli.cpp -- cgmlst-hamming-gpu: drop-in command line for the MI355X library.
+//
+// Same flags, defaults, output bytes and exit codes as the reference host
+// (tensorcore/rs/src/main.rs:19-82 flags, 267-491 main): allele-call TSV in,
+// full / lower-tri / upper-tri u16 distance TSV out (gzip with -z or *.gz).
+// Parsing, filters and dense ranks run on the host; the distances run on the
+// GPU through the library's tch_run, exactly the boundary the reference host
+// binds (main.rs:84-93).
+
+#include <errno.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+#include <zlib.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "cgmlst_hamming.h"
+
+namespace {
+
+enum class Layout { Full, LowerTri, UpperTri };
+
+struct Args {
+    std::string input;
+    bool have_input = false;
+    std::string output;
+    bool have_output = false;
+    size_t threads = 0;
+    Layout layout = Layout::Full;
+    std::string input_sep = "\t";
+    std::string output_sep = "\t";
+    std::string index_name = "cgmlst-hamming-gpu";
+    std::string missing = "-";
+    bool skip_repl = false;
+    bool have_lc = false, have_sc = false;
+    double lc = 0, sc = 0;
+    bool gzip = false;
+    int32_t max_dist = 0;
+    size_t tau = 0;
+    size_t stripe = 512;
+    size_t device = 0;
+    bool silent = false;
+};
+
+const char* VERSION = "0.1.0";
+
+[[noreturn]] void die(const std::string& m) {
+    fprintf(stderr, "error: %s\n", m.c_str());
+    exit(1);
+}
+
+[[noreturn]] void usage_error(const std::string& m) {
+    fprintf(stderr, "error: %s\n\nUsage: cgmlst-hamming-gpu [OPTIONS] --input <INPUT>\n\nFor more information, try '--help'.\n",
+            m.c_str());
+    exit(2);
+}
+
+void print_help() {
+    printf(
+        "Exact cgMLST Hamming distance matrix on MI355X matrix cores\n\n"
+        "Usage: cgmlst-hamming-gpu [OPTIONS] --input <INPUT>\n\n"
+        "Options:\n"
+        "  -i, --input <INPUT>                Input allele-call TSV (chewBBACA format). Use '-' for stdin\n"
+        "  -o, --output <OUTPUT>              Output file [default: stdout]\n"
+        "  -t, --threads <THREADS>            Number of CPU threads [default: all cores] [default: 0]\n"
+        "  -m, --matrix-format <MATRIX_FORMAT>  Output matrix layout [default: full] [possible values: full, lower-tri, upper-tri]\n"
+        "  -d, --input-sep <INPUT_SEP>        Input field separator [default: \"\\t\"]\n"
+        "  -D, --output-sep <OUTPUT_SEP>      Output field separator [default: \"\\t\"]\n"
+        "  -x, --index-name <INDEX_NAME>      Name for the top-left index cell [default: cgmlst-hamming-gpu]\n"
+        "  -n, --missing-char <MISSING_CHAR>  Character used for missing data [default: -]\n"
+        "  -r, --skip-input-replacements      Do not strip the chewBBACA `INF-` prefix\n"
+        "  -L, --locus-completeness <LOCUS_COMPLETENESS>    Keep only loci with at least this %% of non-missing calls (0-100)\n"
+        "  -S, --sample-completeness <SAMPLE_COMPLETENESS>  Keep only samples with at least this %% of non-missing calls (0-100)\n"
+        "  -z, --gzip                         gzip-compress the output (also enabled when -o ends with .gz)\n"
+        "  -X, --max-dist <MAX_DIST>          Cap distances at this value (0 = no cap) [default: 0]\n"
+        "      --tau <TAU>                    Frequency threshold: alleles seen >= tau times go to the MFMA GEMM [0 = auto] [default: 0]\n"
+        "      --stripe <STRIPE>              Rows per GPU stripe [default: 512]\n"
+        "      --device <DEVICE>              GPU device [default: 0]\n"
+        "  -s, --silent                       Suppress progress/log messages on stderr\n"
+        "  -h, --help                         Print help\n"
+        "  -V, --version                      Print version\n");
+}
+
+bool parse_usize(const std::string& s, size_t* v) {
+    if (s.empty()) return false;
+    size_t x = 0;
+    for (char c : s) {
+        if (c < '0' || c > '9') return false;
+        if (x > (SIZE_MAX - 9) / 10) return false;
+        x = x * 10 + (size_t)(c - '0');
+    }
+    *v = x;
+    return true;
+}
+
+bool parse_i32(const std::string& s, int32_t* v) {
+    if (s.empty()) return false;
+    size_t i = 0;
+    bool neg = false;
+    if (s[0] == '-' || s[0] == '+') {
+        neg = s[0] == '-';
+        i = 1;
+    }
+    if (i >= s.size()) return false;
+    int64_t x = 0;
+    for (; i < s.size(); i++) {
+        if (s[i] < '0' || s[i] > '9') return false;
+        x = x * 10 + (s[i] - '0');
+        if (x > 2147483648LL) return false;
+    }
+    if (neg) x = -x;
+    if (x > 2147483647LL || x < -2147483648LL) return false;
+    *v = (int32_t)x;
+    return true;
+}
+
+bool parse_f64(const std::string& s, double* v) {
+    if (s.empty()) return false;
+    char* end = NULL;
+    errno = 0;
+    const double x = strtod(s.c_str(), &end);
+    if (*end || end == s.c_str()) return false;
+    *v = x;
+    return true;
+}
+
+Args parse_args(int argc, char** argv) {
+    Args a;
+    struct Opt {
+        char s;
+        const char* l;
+        bool takes;
+        const char* meta;
+    };
+    static const Opt opts[] = {
+        {'i', "input", true, "--input <INPUT>"},
+        {'o', "output", true, "--output <OUTPUT>"},
+        {'t', "threads", true, "--threads <THREADS>"},
+        {'m', "matrix-format", true, "--matrix-format <MATRIX_FORMAT>"},
+        {'d', "input-sep", true, "--input-sep <INPUT_SEP>"},
+        {'D', "output-sep", true, "--output-sep <OUTPUT_SEP>"},
+        {'x', "index-name", true, "--index-name <INDEX_NAME>"},
+        {'n', "missing-char", true, "--missing-char <MISSING_CHAR>"},
+        {'r', "skip-input-replacements", false, "--skip-input-replacements"},
+        {'L', "locus-completeness", true, "--locus-completeness <LOCUS_COMPLETENESS>"},
+        {'S', "sample-completeness", true, "--sample-completeness <SAMPLE_COMPLETENESS>"},
+        {'z', "gzip", false, "--gzip"},
+        {'X', "max-dist", true, "--max-dist <MAX_DIST>"},
+        {0, "tau", true, "--tau <TAU>"},
+        {0, "stripe", true, "--stripe <STRIPE>"},
+        {0, "device", true, "--device <DEVICE>"},
+        {'s', "silent", false, "--silent"},
+        {'h', "help", false, "--help"},
+        {'V', "version", false, "--version"},
+    };
+    auto apply = [&](const Opt& o, const std::string& v) {
+        const std::string name = o.l;
+        auto bad = [&]() { usage_error("invalid value '" + v + "' for '" + o.meta + "'"); };
+        if (name == "input") { a.input = v; a.have_input = true; }
+        else if (name == "output") { a.output = v; a.have_output = true; }
+        else if (name == "threads") { if (!parse_usize(v, &a.threads)) bad(); }
+        else if (name == "matrix-format") {
+            if (v == "full") a.layout = Layout::Full;
+            else if (v == "lower-tri") a.layout = Layout::LowerTri;
+            else if (v == "upper-tri") a.layout = Layout::UpperTri;
+            else usage_error("invalid value '" + v + "' for '" + o.meta + "'\n  [possible values: full, lower-tri, upper-tri]");
+        }
+        else if (name == "input-sep") a.input_sep = v;
+        else if (name == "output-sep") a.output_sep = v;
+        else if (name == "index-name") a.index_name = v;
+        else if (name == "missing-char") a.missing = v;
+        else if (name == "skip-input-replacements") a.skip_repl = true;
+        else if (name == "locus-completeness") { if (!parse_f64(v, &a.lc)) bad(); a.have_lc = true; }
+        else if (name == "sample-completeness") { if (!parse_f64(v, &a.sc)) bad(); a.have_sc = true; }
+        else if (name == "gzip") a.gzip = true;
+        else if (name == "max-dist") { if (!parse_i32(v, &a.max_dist)) bad(); }
+        else if (name == "tau") { if (!parse_usize(v, &a.tau)) bad(); }
+        else if (name == "stripe") { if (!parse_usize(v, &a.stripe)) bad(); }
+        else if (name == "device") { if (!parse_usize(v, &a.device)) bad(); }
+        else if (name == "silent") a.silent = true;
+        else if (name == "help") { print_help(); exit(0); }
+        else if (name == "version") { printf("cgmlst-hamming-gpu %s\n", VERSION); exit(0); }
+    };
+    for (int k = 1; k < argc; k++) {
+        std::string t = argv[k];
+        if (t.size() > 2 && t[0] == '-' && t[1] == '-') {
+            std::string name = t.substr(2), val;
+            bool has_val = false;
+            const size_t eq = name.find('=');
+            if (eq != std::string::npos) {
+                val = name.substr(eq + 1);
+                name = name.substr(0, eq);
+                has_val = true;
+            }
+            const Opt* o = NULL;
+            for (auto& x : opts)
+                if (name == x.l) o = &x;
+            if (!o) usage_error("unexpected argument '" + t + "' found");
+            if (o->takes) {
+                if (!has_val) {
+                    if (k + 1 >= argc) usage_error(std::string("a value is required for '") + o->meta + "' but none was supplied");
+                    val = argv[++k];
+                }
+                apply(*o, val);
+            } else {
+                if (has_val) usage_error("unexpected value '" + val + "' for '" + o->meta + "' found; no more were expected");
+                apply(*o, "");
+            }
+        } else if (t.size() > 1 && t[0] == '-' && t != "--") {
+            for (size_t p = 1; p < t.size(); p++) {
+                const Opt* o = NULL;
+                for (auto& x : opts)
+                    if (x.s && x.s == t[p]) o = &x;
+                if (!o) usage_error(std::string("unexpected argument '-") + t[p] + "' found");
+                if (!o->takes) {
+                    apply(*o, "");
+                    continue;
+                }
+                std::string val = t.substr(p + 1);
+                if (!val.empty() && val[0] == '=') val = val.substr(1);
+                if (val.empty() && p + 1 == t.size()) {
+                    if (k + 1 >= argc) usage_error(std::string("a value is required for '") + o->meta + "' but none was supplied");
+                    val = argv[++k];
+                }
+                apply(*o, val);
+                break;
+            }
+        } else {
+            usage_error("unexpected argument '" + t + "' found");
+        }
+    }
+    if (!a.have_input) usage_error("the following required arguments were not provided:\n  --input <INPUT>");
+    return a;
+}
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Rust str::trim(): strip Unicode White_Space from both ends (UTF-8 input).
+bool ws_at(const char* s, size_t len, size_t pos, size_t* clen) {
+    const unsigned char c = (unsigned char)s[pos];
+    if (c == ' ' || (c >= 0x09 && c <= 0x0d)) { *clen = 1; return true; }
+    if (c == 0xc2 && pos + 1 < len) {
+        const unsigned char d = (unsigned char)s[pos + 1];
+        if (d == 0x85 || d == 0xa0) { *clen = 2; return true; }
+    }
+    if ((c == 0xe1 || c == 0xe2 || c == 0xe3) && pos + 2 < len) {
+        const unsigned char d = (unsigned char)s[pos + 1], e = (unsigned char)s[pos + 2];
+        const uint32_t cp = ((uint32_t)(c & 0x0f) << 12) | ((uint32_t)(d & 0x3f) << 6) | (e & 0x3f);
+        if (cp == 0x1680 || (cp >= 0x2000 && cp <= 0x200a) || cp == 0x2028 || cp == 0x2029 || cp == 0x202f ||
+            cp == 0x205f || cp == 0x3000) {
+            *clen = 3;
+            return true;
+        }
+    }
+    return false;
+}
+
+size_t utf8_start_of_last(const char* s, size_t end) {
+    size_t p = end - 1;
+    while (p > 0 && ((unsigned char)s[p] & 0xc0) == 0x80) p--;
+    return p;
+}
+
+void trim(const char*& b, const char*& e) {
+    size_t cl;
+    while (b < e && ws_at(b, (size_t)(e - b), 0, &cl)) b += cl;
+    while (b < e) {
+        const size_t p = utf8_start_of_last(b, (size_t)(e - b));
+        if (ws_at(b, (size_t)(e - b), p, &cl) && p + cl == (size_t)(e - b)) e = b + p;
+        else break;
+    }
+}
+
+// main.rs:97-113.  Returns false on u64 overflow.
+bool parse_allele(const char* b, const char* e, bool skip_repl, const std::string& missing, uint64_t* out) {
+    trim(b, e);
+    if (!skip_repl && e - b >= 4 && memcmp(b, "INF-", 4) == 0) b += 4;
+    const size_t len = (size_t)(e - b);
+    if ((len == missing.size() && memcmp(b, missing.data(), len) == 0) || len == 0) {
+        *out = 0;
+        return true;
+    }
+    for (const char* p = b; p < e; p++)
+        if (*p < '0' || *p > '9') {
+            *out = 0;
+            return true;
+        }
+    uint64_t x = 0;
+    for (const char* p = b; p < e; p++) {
+        const uint64_t d = (uint64_t)(*p - '0');
+        if (x > (UINT64_MAX - d) / 10) return false;
+        x = x * 10 + d;
+    }
+    *out = x;
+    return true;
+}
+
+bool valid_utf8(const std::string& s) {
+    const unsigned char* p = (const unsigned char*)s.data();
+    const size_t n = s.size();
+    size_t i = 0;
+    while (i < n) {
+        const unsigned char c = p[i];
+        if (c < 0x80) { i++; continue; }
+        size_t len;
+        uint32_t cp;
+        if ((c & 0xe0) == 0xc0) { len = 2; cp = c & 0x1f; }
+        else if ((c & 0xf0) == 0xe0) { len = 3; cp = c & 0x0f; }
+        else if ((c & 0xf8) == 0xf0) { len = 4; cp = c & 0x07; }
+        else return false;
+        if (i + len > n) return false;
+        for (size_t k = 1; k < len; k++) {
+            if ((p[i + k] & 0xc0) != 0x80) return false;
+            cp = (cp << 6) | (p[i + k] & 0x3f);
+        }
+        if ((len == 2 && cp < 0x80) || (len == 3 && cp < 0x800) || (len == 4 && (cp < 0x10000 || cp > 0x10ffff)) ||
+            (cp >= 0xd800 && cp <= 0xdfff))
+            return false;
+        i += len;
+    }
+    return true;
+}
+
+std::string io_error(int e) { return std::string(strerror(e)) + " (os error " + std::to_string(e) + ")"; }
+
+bool read_all(const std::string& path, std::string* out, int* eno) {
+    FILE* f = path == "-" ? stdin : fopen(path.c_str(), "rb");
+    if (!f) {
+        *eno = errno;
+        return false;
+    }
+    char buf[1 << 16];
+    size_t r;
+    while ((r = fread(buf, 1, sizeof buf, f)) > 0) out->append(buf, r);
+    const bool ok = !ferror(f);
+    *eno = errno;
+    if (f != stdin) fclose(f);
+    return ok;
+}
+
+struct Upper {
+    std::vector<std::vector<uint16_t>> rows;  // row i: H[i][i+1..n-1]
+};
+
+void on_stripe(void* ctx, int r0, int rows, size_t w, const uint16_t* stripe) {
+    Upper* u = (Upper*)ctx;
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < rows; r++)
+        u->rows[(size_t)r0 + r].assign(stripe + (size_t)r * w + r + 1, stripe + (size_t)(r + 1) * w);
+}
+
+inline uint16_t at(const Upper& u, size_t i, size_t j) {
+    if (i == j) return 0;
+    if (j > i) return u.rows[i][j - i - 1];
+    return u.rows[j][i - j - 1];
+}
+
+struct Sink {
+    FILE* f = NULL;
+    gzFile z = NULL;
+    void write(const char* p, size_t len) {
+        if (!len) return;
+        if (z) {
+            if (gzwrite(z, p, (unsigned)len) != (int)len) die("writing output: gzip error");
+        } else if (fwrite(p, 1, len, f) != len) {
+            die("writing output: " + io_error(errno));
+        }
+    }
+};
+
+char* fmt_u(char* p, uint32_t v) {
+    char tmp[12];
+    int k = 0;
+    do {
+        tmp[k++] = (char)('0' + v % 10);
+        v /= 10;
+    } while (v);
+    while (k) *p++ = tmp[--k];
+    return p;
+}
+
+// main.rs:225-265: header, then rows formatted in parallel chunks of 1024.
+void write_matrix(Sink& out, const std::vector<std::string>& ids, const Upper& up, const std::string& osep,
+                  const std::string& index_name, Layout layout, int32_t cap) {
+    const size_t n = ids.size();
+    std::string head = index_name;
+    for (auto& id : ids) {
+        head += osep;
+        head += id;
+    }
+    head += "\n";
+    out.write(head.data(), head.size());
+    const size_t CHUNK = 1024;
+    std::vector<std::string> bufs(CHUNK);
+    for (size_t s = 0; s < n; s += CHUNK) {
+        const size_t e = std::min(n, s + CHUNK);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (long ii = (long)s; ii < (long)e; ii++) {
+            const size_t i = (size_t)ii;
+            std::string& b = bufs[i - s];
+            b.resize(ids[i].size() + n * (osep.size() + 11) + 1);
+            char* p = &b[0];
+            memcpy(p, ids[i].data(), ids[i].size());
+            p += ids[i].size();
+            for (size_t j = 0; j < n; j++) {
+                memcpy(p, osep.data(), osep.size());
+                p += osep.size();
+                bool keep = layout == Layout::Full || (layout == Layout::LowerTri ? j <= i : j >= i);
+                int32_t v = keep ? (int32_t)at(up, i, j) : 0;
+                if (cap > 0 && v > cap) v = cap;
+                p = fmt_u(p, (uint32_t)v);
+            }
+            *p++ = '\n';
+            b.resize((size_t)(p - &b[0]));
+        }
+        for (size_t i = s; i < e; i++) out.write(bufs[i - s].data(), bufs[i - s].size());
+    }
+}
+
+void log_line(void* ctx, const char* line) {
+    if (!*(bool*)ctx) fprintf(stderr, "%s\n", line);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Args args = parse_args(argc, argv);
+    const double t_start = now();
+    auto log = [&](const std::string& m) {
+        if (!args.silent) fprintf(stderr, "%s\n", m.c_str());
+    };
+    char msg[512];
+    if (args.threads > 0) omp_set_num_threads((int)args.threads);
+    if (args.input_sep.size() != 1) die("input separator must be a single character");
+    const char sep = args.input_sep[0];
+    if ((size_t)std::max(tch_device_count(), 0) <= args.device)
+        die("GPU device " + std::to_string(args.device) + " not available");
+
+    // ---- load (main.rs:286-311) ----
+    double t = now();
+    std::string text;
+    int eno = 0;
+    if (!read_all(args.input, &text, &eno)) die("reading " + args.input + ": " + io_error(eno));
+    if (!valid_utf8(text)) die("reading " + args.input + ": stream did not contain valid UTF-8");
+    // Rust str::lines(): split on '\n', drop one trailing '\r' per line, no
+    // final empty line after a trailing '\n'.
+    std::vector<std::pair<size_t, size_t>> lines;
+    for (size_t p = 0; p < text.size();) {
+        size_t q = text.find('\n', p);
+        const size_t end = q == std::string::npos ? text.size() : q;
+        size_t e = end;
+        if (e > p && text[e - 1] == '\r') e--;
+        lines.emplace_back(p, e);
+        p = q == std::string::npos ? text.size() : q + 1;
+    }
+    if (lines.empty()) die("empty input");
+    size_t l = 0;
+    for (size_t p = lines[0].first; p < lines[0].second; p++) l += text[p] == sep;
+    std::vector<std::pair<size_t, size_t>> data;
+    for (size_t k = 1; k < lines.size(); k++)
+        if (lines[k].second > lines[k].first) data.push_back(lines[k]);
+    size_t n = data.size();
+    std::vector<std::string> ids(n);
+    std::vector<uint64_t> calls(n * l, 0);
+    {
+        std::vector<std::string> errs(n);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long ii = 0; ii < (long)n; ii++) {
+            const size_t i = (size_t)ii;
+            const char* b = text.data() + data[i].first;
+            const char* e = text.data() + data[i].second;
+            const char* f = (const char*)memchr(b, sep, (size_t)(e - b));
+            const char* idend = f ? f : e;
+            ids[i].assign(b, idend);
+            if (!f) continue;
+            const char* c = f + 1;
+            size_t k = 0;
+            while (true) {
+                const char* ce = (const char*)memchr(c, sep, (size_t)(e - c));
+                if (!ce) ce = e;
+                uint64_t v;
+                if (!parse_allele(c, ce, args.skip_repl, args.missing, &v)) {
+                    const char* tb = c;
+                    const char* te = ce;
+                    trim(tb, te);
+                    errs[i] = "sample " + ids[i] + ": allele '" + std::string(tb, te) + "' too large";
+                    break;
+                }
+                if (k < l) calls[i * l + k] = v;
+                k++;
+                if (ce == e) break;
+                c = ce + 1;
+            }
+        }
+        for (auto& e : errs)
+            if (!e.empty()) die(e);
+    }
+    snprintf(msg, sizeof msg, "Loaded %zu samples x %zu loci in %.2fs", n, l, now() - t);
+    log(msg);
+
+    // ---- completeness filters (main.rs:313-351) ----
+    if (args.have_lc) {
+        std::vector<size_t> keep;
+        for (size_t k = 0; k < l; k++) {
+            size_t c = 0;
+            for (size_t i = 0; i < n; i++) c += calls[i * l + k] != 0;
+            if (((double)c / (double)n) * 100.0 >= args.lc) keep.push_back(k);
+        }
+        if (keep.size() != l) {
+            std::vector<uint64_t> nc(n * keep.size());
+            for (size_t i = 0; i < n; i++)
+                for (size_t d = 0; d < keep.size(); d++) nc[i * keep.size() + d] = calls[i * l + keep[d]];
+            calls.swap(nc);
+            l = keep.size();
+        }
+        snprintf(msg, sizeof msg, "Locus completeness >= %g%%: kept %zu loci", args.lc, l);
+        log(msg);
+    }
+    if (args.have_sc) {
+        std::vector<size_t> keep;
+        for (size_t i = 0; i < n; i++) {
+            size_t c = 0;
+            for (size_t k = 0; k < l; k++) c += calls[i * l + k] != 0;
+            if (((double)c / (double)l) * 100.0 >= args.sc) keep.push_back(i);
+        }
+        if (keep.size() != n) {
+            std::vector<uint64_t> nc(keep.size() * l);
+            std::vector<std::string> nids;
+            for (size_t d = 0; d < keep.size(); d++) {
+                std::copy(calls.begin() + keep[d] * l, calls.begin() + (keep[d] + 1) * l, nc.begin() + d * l);
+                nids.push_back(ids[keep[d]]);
+            }
+            calls.swap(nc);
+            ids.swap(nids);
+            n = keep.size();
+        }
+        snprintf(msg, sizeof msg, "Sample completeness >= %g%%: kept %zu samples", args.sc, n);
+        log(msg);
+    }
+    if (l > 65535) die("more than 65535 loci (u16 distances)");
+
+    // ---- encode (main.rs:353-445) ----
+    t = now();
+    char err[512] = {0};
+    std::vector<uint16_t> X(n * l);
+    if (cgh_dense_ranks(calls.data(), (int)n, (int)l, X.data(), err, sizeof err)) die(err);
+    std::vector<uint64_t>().swap(calls);
+    Upper up;
+    up.rows.resize(n);
+    if (n >= 2 && l > 0) {
+        const size_t stripe = std::max<size_t>(args.stripe, 8) / 8 * 8;
+        size_t tau = args.tau;
+        if (tau == 0) {
+            std::vector<uint32_t> cnt(cgh_allele_counts(X.data(), (int)n, (int)l, NULL, 0));
+            cgh_allele_counts(X.data(), (int)n, (int)l, cnt.data(), cnt.size());
+            const size_t freeb = tch_free_vram((int)args.device);
+            tau = (size_t)cgh_choose_tau(cnt.data(), cnt.size(), (int)n, (int)l, (int)stripe, freeb, log_line,
+                                         &args.silent);
+            if (tau == 0) {
+                snprintf(msg, sizeof msg, "no tau fits in %.2f GB of free VRAM", (double)freeb / 1e9);
+                die(msg);
+            }
+        }
+        const int tau_i = (int)std::min<size_t>(tau, (size_t)n + 1);  // counts never exceed n
+        cgh_encoding enc;
+        if (cgh_encode(X.data(), (int)n, (int)l, tau_i, 1024, &enc, err, sizeof err)) die(err);
+        if (enc.n_rare >= (1ull << 32)) die("too many rare pairs: raise --tau");
+        std::vector<uint32_t> start(n + 1), js(std::max<uint64_t>(enc.n_rare, 1));
+        if (cgh_rare_csr(&enc, start.data(), js.data(), err, sizeof err)) die(err);
+        snprintf(msg, sizeof msg, "Encoded in %.2fs: tau=%zu, one-hot cols=%zu, rare equal pairs=%llu", now() - t, tau,
+                 enc.D, (unsigned long long)enc.n_rare);
+        log(msg);
+
+        // ---- GPU (main.rs:447-465) ----
+        t = now();
+        double gpu_secs = 0;
+        int rare_gpu = 0;
+        const int rc = tch_run((int)args.device, (int)n, enc.n_pad, (const uint8_t*)enc.O, enc.kw1 * 32,
+                               (const uint8_t*)enc.P, enc.kw2 * 32, start.data(), js.data(), (size_t)enc.n_rare,
+                               (int)stripe, on_stripe, &up, &gpu_secs, &rare_gpu, err, sizeof err);
+        cgh_encoding_free(&enc);
+        if (rc) die(std::string("GPU: ") + err);
+        snprintf(msg, sizeof msg,
+                 "Computed %zu x %zu matrix on MFMA in %.2fs (GPU pipeline %.2fs, rare pairs on %s)", n, n,
+                 now() - t, gpu_secs, rare_gpu ? "GPU" : "host");
+        log(msg);
+    } else {
+        for (size_t i = 0; i < n; i++) up.rows[i].assign(n - 1 - i, 0);
+    }
+
+    // ---- write (main.rs:472-490) ----
+    t = now();
+    const bool gz = args.gzip || (args.have_output && args.output.size() >= 3 &&
+                                  args.output.compare(args.output.size() - 3, 3, ".gz") == 0);
+    Sink sink;
+    if (args.have_output) {
+        sink.f = fopen(args.output.c_str(), "wb");
+        if (!sink.f) die("creating " + args.output + ": " + io_error(errno));
+    } else {
+        sink.f = stdout;
+    }
+    if (gz) {
+        sink.z = gzdopen(dup(fileno(sink.f)), "wb3");
+        if (!sink.z) die("creating gzip stream");
+    } else {
+        setvbuf(sink.f, NULL, _IOFBF, 16 << 20);
+    }
+    write_matrix(sink, ids, up, args.output_sep, args.index_name, args.layout, args.max_dist);
+    if (sink.z && gzclose(sink.z) != Z_OK) die("writing output: gzip error");
+    if (fflush(sink.f) != 0) die("writing output: " + io_error(errno));
+    if (sink.f != stdout) fclose(sink.f);
+    snprintf(msg, sizeof msg, "Wrote output in %.2fs. Total %.2fs", now() - t, now() - t_start);
+    log(msg);
+    return 0;
+}

@@ -1,0 +1,293 @@
+// encode.cpp -- host encoder of the exact Hamming identity (C++17 + OpenMP).
+//
+// Dense ranks, allele counts, the frequent-column map, O/P bit packing, rare
+// groups, the reference's rare CSR and the tau cost model.  All of it is
+// integer bookkeeping on the host; the device kernels consume its output.
+//
+// Reference lines restated: tensorcore/rs/src/main.rs:135-165 (dense_ranks),
+// 169-198 (choose_tau), 366-385 (column map), 386-403 (bit pack), 404-444
+// (rare groups and CSR).
+
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <vector>
+
+#include "cgmlst_hamming.h"
+
+namespace {
+
+void set_err(char* err, int errlen, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+void set_err(char* err, int errlen, const char* fmt, ...) {
+    if (!err || errlen <= 0) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, (size_t)errlen, fmt, ap);
+    va_end(ap);
+}
+
+size_t up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// Per-locus counts of ranks 0..max (cnt[k][0] = missing), like main.rs:146-153.
+void locus_counts(const uint16_t* X, int n, int L, std::vector<std::vector<uint32_t>>& cnt) {
+    cnt.assign((size_t)L, {});
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < L; k++) {
+            uint32_t m = 0;
+            for (int i = 0; i < n; i++) m = std::max<uint32_t>(m, X[(size_t)i * L + k]);
+            std::vector<uint32_t> c(m + 1, 0);
+            for (int i = 0; i < n; i++) c[X[(size_t)i * L + k]]++;
+            cnt[k].swap(c);
+        }
+}
+
+}  // namespace
+
+extern "C" int cgh_dense_ranks(const uint64_t* calls, int n, int L, uint16_t* X, char* err, int errlen) {
+    std::atomic<int> bad_locus{-1};
+    std::atomic<size_t> bad_count{0};
+#pragma omp parallel
+    {
+        std::vector<uint64_t> u;
+        u.reserve((size_t)n);
+#pragma omp for schedule(dynamic, 4)
+        for (int k = 0; k < L; k++) {
+            u.clear();
+            for (int i = 0; i < n; i++) {
+                const uint64_t v = calls[(size_t)i * L + k];
+                if (v) u.push_back(v);
+            }
+            std::sort(u.begin(), u.end());
+            u.erase(std::unique(u.begin(), u.end()), u.end());
+            if (u.size() > 65535) {
+                int expect = -1;
+                if (bad_locus.compare_exchange_strong(expect, k)) bad_count = u.size();
+                continue;
+            }
+            for (int i = 0; i < n; i++) {
+                const uint64_t v = calls[(size_t)i * L + k];
+                X[(size_t)i * L + k] =
+                    v ? (uint16_t)(std::lower_bound(u.begin(), u.end(), v) - u.begin() + 1) : (uint16_t)0;
+            }
+        }
+    }
+    if (bad_locus >= 0) {
+        set_err(err, errlen, "locus %d has %zu distinct alleles (limit 65535)", bad_locus.load(), bad_count.load());
+        return 1;
+    }
+    return 0;
+}
+
+extern "C" size_t cgh_allele_counts(const uint16_t* X, int n, int L, uint32_t* cnt, size_t cap) {
+    std::vector<std::vector<uint32_t>> c;
+    locus_counts(X, n, L, c);
+    size_t need = 0;
+    for (auto& v : c) need += v.size() - 1;
+    if (cnt && cap >= need) {
+        size_t o = 0;
+        for (auto& v : c)
+            for (size_t a = 1; a < v.size(); a++) cnt[o++] = v[a];
+    }
+    return need;
+}
+
+extern "C" void cgh_encoding_free(cgh_encoding* enc) {
+    if (!enc) return;
+    free(enc->O);
+    free(enc->P);
+    free(enc->gstart);
+    free(enc->gmem);
+    free(enc->gpairs);
+    memset(enc, 0, sizeof *enc);
+}
+
+extern "C" int cgh_encode(const uint16_t* X, int n, int L, int tau, int k_align, cgh_encoding* enc, char* err,
+                          int errlen) {
+    memset(enc, 0, sizeof *enc);
+    if (n < 0 || L < 0 || tau < 1 || (k_align != 256 && k_align != 512 && k_align != 1024)) {
+        set_err(err, errlen, "cgh_encode: bad arguments (n=%d L=%d tau=%d k_align=%d)", n, L, tau, k_align);
+        return 1;
+    }
+    std::vector<std::vector<uint32_t>> cnt;
+    locus_counts(X, n, L, cnt);
+    // frequent-column map (main.rs:366-385): col[k][a] local index or -1
+    std::vector<std::vector<int32_t>> col((size_t)L);
+    std::vector<size_t> base((size_t)L + 1, 0);
+    for (int k = 0; k < L; k++) {
+        col[k].assign(cnt[k].size(), -1);
+        int32_t c = 0;
+        for (size_t a = 1; a < cnt[k].size(); a++)
+            if (cnt[k][a] >= (uint32_t)tau) col[k][a] = c++;
+        base[k + 1] = base[k] + (size_t)c;
+    }
+    const size_t D = base[L];
+    enc->n = n;
+    enc->L = L;
+    enc->tau = tau;
+    enc->D = D;
+    enc->n_pad = up((size_t)std::max(n, 1), 256);
+    enc->kw1 = up(std::max<size_t>(D, 1), (size_t)k_align) / 32;
+    enc->kw2 = up(std::max<size_t>((size_t)L, 1), (size_t)k_align) / 32;
+    enc->O = (uint32_t*)calloc(enc->n_pad * enc->kw1, 4);
+    enc->P = (uint32_t*)calloc(enc->n_pad * enc->kw2, 4);
+    if (!enc->O || !enc->P) {
+        cgh_encoding_free(enc);
+        set_err(err, errlen, "out of host memory for the bit matrices");
+        return 1;
+    }
+    // bit pack (main.rs:386-403), LSB-first
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        uint32_t* orow = enc->O + (size_t)i * enc->kw1;
+        uint32_t* prow = enc->P + (size_t)i * enc->kw2;
+        for (int k = 0; k < L; k++) {
+            const uint16_t a = X[(size_t)i * L + k];
+            if (!a) continue;
+            prow[k >> 5] |= 1u << (k & 31);
+            const int32_t c = col[k][a];
+            if (c >= 0) {
+                const size_t cc = base[k] + (size_t)c;
+                orow[cc >> 5] |= 1u << (cc & 31);
+            }
+        }
+    }
+    // rare groups (main.rs:404-417): (locus, allele) with 2 <= count < tau,
+    // ordered by locus then rank, members ascending
+    std::vector<size_t> lg((size_t)L + 1, 0), lm((size_t)L + 1, 0);
+    for (int k = 0; k < L; k++) {
+        size_t g = 0, m = 0;
+        for (size_t a = 1; a < cnt[k].size(); a++)
+            if (col[k][a] < 0 && cnt[k][a] >= 2) {
+                g++;
+                m += cnt[k][a];
+            }
+        lg[k + 1] = lg[k] + g;
+        lm[k + 1] = lm[k] + m;
+    }
+    enc->n_groups = lg[L];
+    enc->n_members = lm[L];
+    if (enc->n_members > 0xffffffffull) {
+        cgh_encoding_free(enc);
+        set_err(err, errlen, "too many rare-group members: raise --tau");
+        return 1;
+    }
+    enc->gstart = (uint32_t*)malloc((enc->n_groups + 1) * 4);
+    enc->gmem = (uint32_t*)malloc(std::max<size_t>(enc->n_members, 1) * 4);
+    enc->gpairs = (uint64_t*)malloc((enc->n_groups + 1) * 8);
+    if (!enc->gstart || !enc->gmem || !enc->gpairs) {
+        cgh_encoding_free(enc);
+        set_err(err, errlen, "out of host memory for the rare groups");
+        return 1;
+    }
+#pragma omp parallel
+    {
+        std::vector<int64_t> slot;   // rank -> write cursor into gmem, -1 if not rare
+#pragma omp for schedule(dynamic, 4)
+        for (int k = 0; k < L; k++) {
+            slot.assign(cnt[k].size(), -1);
+            size_t g = lg[k], m = lm[k];
+            for (size_t a = 1; a < cnt[k].size(); a++)
+                if (col[k][a] < 0 && cnt[k][a] >= 2) {
+                    enc->gstart[g] = (uint32_t)m;
+                    const uint64_t c = cnt[k][a];
+                    enc->gpairs[g] = c * (c - 1) / 2;  // per-group count, prefixed below
+                    slot[a] = (int64_t)m;
+                    g++;
+                    m += c;
+                }
+            for (int i = 0; i < n; i++) {
+                const uint16_t a = X[(size_t)i * L + k];
+                if (a && slot[a] >= 0) enc->gmem[slot[a]++] = (uint32_t)i;
+            }
+        }
+    }
+    enc->gstart[enc->n_groups] = (uint32_t)enc->n_members;
+    uint64_t acc = 0;
+    for (size_t g = 0; g < enc->n_groups; g++) {
+        const uint64_t c = enc->gpairs[g];
+        enc->gpairs[g] = acc;
+        acc += c;
+    }
+    enc->gpairs[enc->n_groups] = acc;
+    enc->n_rare = acc;
+    return 0;
+}
+
+extern "C" int cgh_rare_csr(const cgh_encoding* enc, uint32_t* start, uint32_t* js, char* err, int errlen) {
+    const int n = enc->n;
+    if (enc->n_rare >= (1ull << 32)) {
+        set_err(err, errlen, "too many rare pairs: raise --tau");
+        return 1;
+    }
+    std::vector<std::atomic<uint64_t>> counts((size_t)n);
+    for (auto& c : counts) c.store(0, std::memory_order_relaxed);
+    const long G = (long)enc->n_groups;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long g = 0; g < G; g++) {
+        const uint32_t s = enc->gstart[g], c = enc->gstart[g + 1] - s;
+        for (uint32_t a = 0; a < c; a++)
+            counts[enc->gmem[s + a]].fetch_add(c - 1 - a, std::memory_order_relaxed);
+    }
+    uint64_t tot = 0;
+    for (int i = 0; i < n; i++) {
+        start[i] = (uint32_t)tot;
+        tot += counts[i].load(std::memory_order_relaxed);
+    }
+    start[n] = (uint32_t)tot;
+    std::vector<std::atomic<uint32_t>> fill((size_t)n);
+    for (int i = 0; i < n; i++) fill[i].store(start[i], std::memory_order_relaxed);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long g = 0; g < G; g++) {
+        const uint32_t s = enc->gstart[g], c = enc->gstart[g + 1] - s;
+        for (uint32_t a = 0; a < c; a++) {
+            const uint32_t i = enc->gmem[s + a];
+            for (uint32_t b = a + 1; b < c; b++) js[fill[i].fetch_add(1, std::memory_order_relaxed)] = enc->gmem[s + b];
+        }
+    }
+    return 0;
+}
+
+// Rates of the MI355X kernels used by the cost model (measured with bench.py
+// on the 63,005 x 1,142 long-tail workload): binary GEMM in pair-bits per
+// second over the triangle, rare enumeration in pairs per second.
+static const double GEMM_RATE = 1.2e15;
+static const double RARE_RATE = 1.0e10;
+
+extern "C" int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, int S, size_t free_bytes,
+                              void (*log)(void* ctx, const char* line), void* log_ctx) {
+    const double n_pad = (double)up((size_t)std::max(n, 1), 256);
+    const double k2 = (double)up((size_t)std::max(L, 1), 1024);
+    const double bufs = (double)S * n_pad * 2.0 * 2.0 + (double)(64u << 20);
+    int best = 0;
+    double best_t = 1e300;
+    for (int tau = 2; tau <= 4096; tau *= 2) {
+        size_t d = 0;
+        double r = 0;
+        for (size_t t = 0; t < ncnt; t++) {
+            const uint32_t x = cnt[t];
+            if (!x) continue;
+            if (x >= (uint32_t)tau) d++;
+            else r += (double)x * ((double)x - 1.0) / 2.0;
+        }
+        const double k1 = (double)up(std::max<size_t>(d, 1), 1024);
+        const double mem = n_pad * (k1 + k2) / 8.0 + 4.0 * (r + n + 1.0) + bufs;
+        const double t = (double)n * (double)n / 2.0 * (k1 + k2) / GEMM_RATE + r / RARE_RATE;
+        const bool fits = mem <= (double)free_bytes && r < 4.0e9;
+        if (log) {
+            char line[256];
+            snprintf(line, sizeof line, "  tau=%5d  one-hot=%8zu  rare=%14.0f  est %.3fs  vram %.2f GB%s", tau, d, r, t,
+                     mem / 1e9, fits ? "" : "  (does not fit)");
+            log(log_ctx, line);
+        }
+        if (fits && t < best_t) {
+            best_t = t;
+            best = tau;
+        }
+    }
+    return best;
+}

@@ -1,0 +1,535 @@
+// hamming.hip -- gfx950 kernels and the device side of the C ABI.
+//
+//   H = P P^T - O O^T - R        (exact, integer arithmetic throughout)
+//
+// The binary products run on int8 MFMA (v_mfma_i32_32x32x32_i8) over 0/1
+// operands expanded on the fly from bit-packed rows: every workgroup stages a
+// KC-word slab of its A rows and B rows in LDS as raw 32-bit words (k-major,
+// so a wavefront reads 32 consecutive rows of one word without bank
+// conflicts), and each wavefront expands a word into the 16 int8 bytes of its
+// MFMA operand with shifts and masks.  The same bit -> (lane half, byte) map
+// is used for A and B, so every bit of the word is counted exactly once.
+// One accumulator carries the whole identity: the O slab is accumulated
+// first, the accumulator is negated, then the P slab is added, so the
+// epilogue holds P P^T - O O^T and writes u16 directly (no int32 stripe, no
+// separate pack pass).  Rare equal pairs are subtracted afterwards with
+// 32-bit atomics on the packed u16 cells (a cell never drops below its final
+// value, so the half-word subtraction never borrows into its neighbour).
+//
+// Reference structure this replaces: tensorcore/rs/cuda/tch.cu:22-66 (CUTLASS
+// 1-bit GEMM, two launches with beta = -1), 36-43 (sub_rare), 45-50
+// (pack_u16), 82-170 (tch_run stripe pipeline).
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <type_traits>
+
+#include "cgmlst_hamming.h"
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int KC = 8;  // 32-bit words per row staged per pipeline step (256 bits)
+
+struct TileArgs {
+    const uint32_t* O;  // n_pad x kw1 words (frequent-allele one-hot bits)
+    const uint32_t* P;  // n_pad x kw2 words (presence bits)
+    int kw1, kw2;       // words per row, multiples of KC
+    int n;              // valid rows
+    int r0, rows;       // first row of this stripe and its row count
+    int w;              // n - r0: output row width, column c <-> j = r0 + c
+    int tiles_n;        // column tiles in this stripe
+    uint16_t* out;      // rows x w u16, row-major
+};
+
+// 32 bits of one row -> 16 int8 operand bytes of lane half g (0 or 1):
+// byte b of dword t is bit 8b + 4g + t.
+__device__ __forceinline__ v4i expand_bits(uint32_t w, int g) {
+    const uint32_t s = w >> (4 * g);
+    v4i r;
+    r[0] = (int)(s & 0x01010101u);
+    r[1] = (int)((s >> 1) & 0x01010101u);
+    r[2] = (int)((s >> 2) & 0x01010101u);
+    r[3] = (int)((s >> 3) & 0x01010101u);
+    return r;
+}
+
+// The same bytes negated: 0x00 / 0xFF (int8 -1).  v_perm_b32 returns 0x00 for
+// selector byte 12 and 0xFF for 13, so the selector is 12 + bit per byte.
+__device__ __forceinline__ v4i expand_bits_neg(uint32_t w, int g) {
+    const uint32_t s = w >> (4 * g);
+    v4i r;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        r[t] = (int)__builtin_amdgcn_perm(0u, 0u, ((s >> t) & 0x01010101u) | 0x0c0c0c0cu);
+    return r;
+}
+
+// Square BM x BM output tile per workgroup, WM x WN per wavefront.
+template <int BM, int WM, int WN>
+__global__ void __launch_bounds__((BM / WM) * (BM / WN) * 64)
+mfma_tile(TileArgs a) {
+    constexpr int WAVES_N = BM / WN;
+    constexpr int NT = (BM / WM) * WAVES_N * 64;
+    constexpr int TM = WM / 32, TN = WN / 32;
+    constexpr int LOADS = 2 * BM * KC / 4;   // 16-byte loads per slab (A and B)
+    constexpr int LPT = LOADS / NT;
+    static_assert(LOADS % NT == 0, "slab loads must divide evenly");
+
+    __shared__ uint32_t sA[2][KC][BM];
+    __shared__ uint32_t sB[2][KC][BM];
+
+    const int tid = threadIdx.x;
+    const int I = blockIdx.x / a.tiles_n, J = blockIdx.x % a.tiles_n;
+    const int rt0 = I * BM, ct0 = J * BM;  // tile origin, stripe-relative
+
+    if (J < I) {  // entirely below the diagonal: every cell has j < i
+        const int rend = min(BM, a.rows - rt0), cend = min(BM, a.w - ct0);
+        for (int e = tid; e < rend * BM; e += NT) {
+            const int r = e / BM, c = e % BM;
+            if (c < cend) a.out[(size_t)(rt0 + r) * a.w + ct0 + c] = 0;
+        }
+        return;
+    }
+
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave / WAVES_N) * WM, wn0 = (wave % WAVES_N) * WN;
+    const int l32 = lane & 31, g = lane >> 5;
+
+    const int nc1 = a.kw1 / KC, nc = nc1 + a.kw2 / KC;
+    const int growA = a.r0 + rt0, growB = a.r0 + ct0;
+
+    uint4 stage[LPT];
+    auto load_slab = [&](int c) {
+        const uint32_t* base;
+        int kw, k0;
+        if (c < nc1) { base = a.O; kw = a.kw1; k0 = c * KC; }
+        else { base = a.P; kw = a.kw2; k0 = (c - nc1) * KC; }
+#pragma unroll
+        for (int u = 0; u < LPT; u++) {
+            const int q = tid + u * NT;
+            const int side = q / (BM * KC / 4);
+            const int qq = q % (BM * KC / 4);
+            const int part = qq / BM, row = qq % BM;
+            const int grow = (side ? growB : growA) + row;
+            if (grow < a.n)
+                stage[u] = *(const uint4*)(base + (size_t)grow * kw + k0 + part * 4);
+            else
+                stage[u] = make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto store_slab = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < LPT; u++) {
+            const int q = tid + u * NT;
+            const int side = q / (BM * KC / 4);
+            const int qq = q % (BM * KC / 4);
+            const int part = qq / BM, row = qq % BM;
+            uint32_t (*s)[BM] = side ? sB[buf] : sA[buf];
+            s[part * 4 + 0][row] = stage[u].x;
+            s[part * 4 + 1][row] = stage[u].y;
+            s[part * 4 + 2][row] = stage[u].z;
+            s[part * 4 + 3][row] = stage[u].w;
+        }
+    };
+
+    v16i acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[i][j][e] = 0;
+
+    load_slab(0);
+    store_slab(0);
+    __syncthreads();
+
+    // O slabs use B = -O (subtracting O O^T), P slabs B = +P.
+    auto compute_slab = [&](int buf, auto neg) {
+#pragma unroll
+        for (int kk = 0; kk < KC; kk++) {
+            v4i av[TM], bv[TN];
+#pragma unroll
+            for (int i = 0; i < TM; i++) av[i] = expand_bits(sA[buf][kk][wm0 + i * 32 + l32], g);
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                const uint32_t wb = sB[buf][kk][wn0 + j * 32 + l32];
+                bv[j] = decltype(neg)::value ? expand_bits_neg(wb, g) : expand_bits(wb, g);
+            }
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+    };
+
+    int c = 0;
+    for (; c < nc1; c++) {
+        const int buf = c & 1;
+        load_slab(c + 1);  // nc1 < nc: the P slabs always follow
+        compute_slab(buf, std::true_type{});
+        store_slab(buf ^ 1);
+        __syncthreads();
+    }
+    for (; c < nc; c++) {
+        const int buf = c & 1;
+        if (c + 1 < nc) load_slab(c + 1);
+        compute_slab(buf, std::false_type{});
+        if (c + 1 < nc) store_slab(buf ^ 1);
+        __syncthreads();
+    }
+
+    // Epilogue: lane holds column l32 and rows 8*(e/4) + 4*g + (e%4).
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            const int c = ct0 + wn0 + j * 32 + l32;
+            if (c >= a.w) continue;
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int r = rt0 + wm0 + i * 32 + (e >> 2) * 8 + g * 4 + (e & 3);
+                if (r < a.rows) a.out[(size_t)r * a.w + c] = (uint16_t)(c > r ? acc[i][j][e] : 0);
+            }
+        }
+}
+
+// Subtract one from the u16 cell at element offset off of a 4-byte aligned
+// u16 array.  The cell never goes below the value it ends with, so the
+// half-word arithmetic inside the 32-bit atomic never borrows.
+__device__ __forceinline__ void sub_one_u16(uint16_t* base, size_t off) {
+    uint32_t* wp = reinterpret_cast<uint32_t*>(base) + (off >> 1);
+    atomicSub(wp, (off & 1) ? 0x10000u : 1u);
+}
+
+// Reference CSR form (tch.cu:36-43): one wavefront per stripe row, lanes
+// stride over the row's partners j > i.
+__global__ void __launch_bounds__(256)
+sub_rare_csr(uint16_t* out, int w, int r0, int rows, const uint32_t* start, const uint32_t* js) {
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (r >= rows) return;
+    const int i = r0 + r;
+    const size_t rowoff = (size_t)r * w - r0;
+    for (uint32_t p = start[i] + lane; p < start[i + 1]; p += 64) sub_one_u16(out, rowoff + js[p]);
+}
+
+// Offset (u16 elements) of stripe s in the concatenated upper-triangle layout.
+__host__ __device__ inline size_t stripe_off(size_t n, size_t S, size_t s) {
+    return S * n * s - S * S * (s * (s - 1) / 2);
+}
+
+// Rare groups form: pairs (a < b) of each group's ascending member list are
+// numbered group by group (prefix gpairs); each thread takes PPT consecutive
+// pair numbers, locates its first pair by binary search and walks on.
+constexpr int PPT = 16;
+__global__ void __launch_bounds__(256)
+sub_rare_groups(uint16_t* out, int n, int S, const uint32_t* gstart, const uint32_t* gmem,
+                const uint64_t* gpairs, size_t n_groups, uint64_t n_rare) {
+    const uint64_t p0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * PPT;
+    if (p0 >= n_rare) return;
+    size_t lo = 0, hi = n_groups - 1;  // last group with gpairs[g] <= p0
+    while (lo < hi) {
+        const size_t mid = (lo + hi + 1) / 2;
+        if (gpairs[mid] <= p0) lo = mid; else hi = mid - 1;
+    }
+    size_t grp = lo;
+    uint64_t q = p0 - gpairs[grp];
+    uint32_t gs = gstart[grp], c = gstart[grp + 1] - gs;
+    // pairs before row a: a*(2c - a - 1)/2; find the largest a with that <= q
+    uint32_t pa = (uint32_t)(((2.0 * c - 1.0) - sqrt((2.0 * c - 1.0) * (2.0 * c - 1.0) - 8.0 * (double)q)) / 2.0);
+    if (pa > c - 2) pa = c - 2;
+    while (pa > 0 && (uint64_t)pa * (2 * c - pa - 1) / 2 > q) pa--;
+    while (pa + 1 <= c - 2 && (uint64_t)(pa + 1) * (2 * c - pa - 2) / 2 <= q) pa++;
+    uint32_t pb = pa + 1 + (uint32_t)(q - (uint64_t)pa * (2 * c - pa - 1) / 2);
+    const uint64_t pend = min((uint64_t)n_rare, p0 + PPT);
+    for (uint64_t p = p0; p < pend; p++) {
+        const uint32_t i = gmem[gs + pa], j = gmem[gs + pb];
+        const size_t s = i / (uint32_t)S;
+        const size_t off = stripe_off(n, S, s) + (size_t)(i - s * S) * (size_t)(n - s * S) + (j - s * S);
+        sub_one_u16(out, off);
+        if (++pb == c) {
+            if (++pa == c - 1) {
+                if (++grp >= n_groups) break;
+                gs = gstart[grp];
+                c = gstart[grp + 1] - gs;
+                pa = 0;
+            }
+            pb = pa + 1;
+        }
+    }
+}
+
+size_t up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+void set_err(char* err, int errlen, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+void set_err(char* err, int errlen, const char* fmt, ...) {
+    if (!err || errlen <= 0) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, (size_t)errlen, fmt, ap);
+    va_end(ap);
+}
+
+#define HK(x)                                                                           \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) {                                                         \
+            set_err(err, errlen, "%s: %s", #x, hipGetErrorString(e_));                  \
+            goto fail;                                                                  \
+        }                                                                               \
+    } while (0)
+
+// Tile shapes: 256 (4 wavefronts of 128 x 128) for large sets, 128 (4
+// wavefronts of 64 x 64) when the 256 grid would leave most CUs idle.
+int pick_tile(int n) { return n >= 8192 ? 256 : 128; }
+
+hipError_t launch_stripe(int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int n, int r0,
+                         int rows, uint16_t* out, hipStream_t st) {
+    TileArgs a;
+    a.O = O; a.P = P; a.kw1 = kw1; a.kw2 = kw2; a.n = n; a.r0 = r0; a.rows = rows; a.w = n - r0;
+    a.out = out;
+    const int tiles_m = (rows + tile - 1) / tile;
+    a.tiles_n = (a.w + tile - 1) / tile;
+    const dim3 grid((unsigned)(tiles_m * a.tiles_n));
+    if (tile == 256)
+        hipLaunchKernelGGL((mfma_tile<256, 128, 128>), grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((mfma_tile<128, 64, 64>), grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Reference-compatible boundary
+// ---------------------------------------------------------------------------
+
+extern "C" int tch_device_count(void) {
+    int c = 0;
+    return hipGetDeviceCount(&c) == hipSuccess ? c : 0;
+}
+
+extern "C" size_t tch_free_vram(int device) {
+    const char* env = getenv("CGH_FREE_BYTES");
+    if (env && *env) return (size_t)strtoull(env, NULL, 10);
+    size_t f = 0, t = 0;
+    if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&f, &t) != hipSuccess) return 0;
+    return f;
+}
+
+extern "C" int tch_run(int device, int n, size_t n_pad, const uint8_t* O, size_t K1, const uint8_t* P,
+                       size_t K2, const uint32_t* start, const uint32_t* js, size_t n_rare, int S, tch_cb cb,
+                       void* ctx, double* gpu_secs, int* rare_on_gpu, char* err, int errlen) {
+    uint32_t *dO = NULL, *dP = NULL, *dStart = NULL, *dJ = NULL;
+    uint16_t *dU[2] = {NULL, NULL}, *hU[2] = {NULL, NULL};
+    hipStream_t st[2] = {NULL, NULL};
+    hipEvent_t done[2] = {NULL, NULL}, t0 = NULL, t1 = NULL;
+    size_t freeb = 0, totb = 0;
+    bool rare_gpu = false;
+    int rc = 1, prev = -1;
+    float ms = 0.f;
+    const int tile = pick_tile(n);
+    const size_t stripe_elems = up((size_t)S * n_pad, 2);
+
+    if (n < 1 || S < 8 || S % 8 || K1 % 1024 || K2 % 1024 || n_pad % 256 || n_pad < (size_t)n) {
+        set_err(err, errlen, "tch_run: bad shape (n=%d n_pad=%zu K1=%zu K2=%zu S=%d)", n, n_pad, K1, K2, S);
+        return 1;
+    }
+    HK(hipSetDevice(device));
+    HK(hipMalloc(&dO, n_pad * K1 / 8));
+    HK(hipMalloc(&dP, n_pad * K2 / 8));
+    HK(hipMemcpy(dO, O, n_pad * K1 / 8, hipMemcpyHostToDevice));
+    HK(hipMemcpy(dP, P, n_pad * K2 / 8, hipMemcpyHostToDevice));
+    HK(hipMemGetInfo(&freeb, &totb));
+    {
+        const char* env = getenv("CGH_FREE_BYTES");
+        if (env && *env) freeb = std::min(freeb, (size_t)strtoull(env, NULL, 10));
+    }
+    // same capacity rule as tch.cu:105, with this pipeline's u16-only stripes
+    rare_gpu = (n_rare + n + 1) * 4 + stripe_elems * 2 * 2 + (64u << 20) < freeb;
+    if (rare_gpu) {
+        HK(hipMalloc(&dStart, (size_t)(n + 1) * 4));
+        HK(hipMalloc(&dJ, std::max<size_t>(n_rare, 1) * 4));
+        HK(hipMemcpy(dStart, start, (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+        if (n_rare) HK(hipMemcpy(dJ, js, n_rare * 4, hipMemcpyHostToDevice));
+    }
+    if (rare_on_gpu) *rare_on_gpu = rare_gpu;
+    for (int b = 0; b < 2; b++) {
+        HK(hipMalloc(&dU[b], stripe_elems * 2));
+        HK(hipHostMalloc(&hU[b], stripe_elems * 2, hipHostMallocDefault));
+        HK(hipStreamCreateWithFlags(&st[b], hipStreamNonBlocking));
+        HK(hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
+    }
+    HK(hipEventCreate(&t0));
+    HK(hipEventCreate(&t1));
+    HK(hipEventRecord(t0, st[0]));
+    for (int r0 = 0;; r0 += S) {
+        if (r0 < n) {
+            const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
+            const size_t w = (size_t)(n - r0);
+            HK(launch_stripe(tile, dO, (int)(K1 / 32), dP, (int)(K2 / 32), n, r0, rows, dU[b], st[b]));
+            if (rare_gpu && n_rare) {
+                hipLaunchKernelGGL(sub_rare_csr, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st[b], dU[b],
+                                   (int)w, r0, rows, dStart, dJ);
+                HK(hipGetLastError());
+            }
+            HK(hipMemcpyAsync(hU[b], dU[b], (size_t)rows * w * 2, hipMemcpyDeviceToHost, st[b]));
+            HK(hipEventRecord(done[b], st[b]));
+        }
+        if (prev >= 0) {  // consume the previous stripe while this one runs
+            const int rows = std::min(S, n - prev), b = (prev / S) & 1;
+            const size_t w = (size_t)(n - prev);
+            HK(hipEventSynchronize(done[b]));
+            if (!rare_gpu)
+                for (int r = 0; r < rows; r++) {
+                    const int i = prev + r;
+                    for (uint32_t p = start[i]; p < start[i + 1]; p++) hU[b][(size_t)r * w + (js[p] - prev)] -= 1;
+                }
+            cb(ctx, prev, rows, w, hU[b]);
+        }
+        if (r0 >= n) break;
+        prev = r0;
+    }
+    HK(hipEventRecord(t1, st[0]));
+    HK(hipEventSynchronize(t1));
+    HK(hipEventElapsedTime(&ms, t0, t1));
+    if (gpu_secs) *gpu_secs = ms / 1e3;
+    rc = 0;
+fail:
+    for (int b = 0; b < 2; b++) {
+        if (dU[b]) (void)hipFree(dU[b]);
+        if (hU[b]) (void)hipHostFree(hU[b]);
+        if (st[b]) (void)hipStreamDestroy(st[b]);
+        if (done[b]) (void)hipEventDestroy(done[b]);
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (dO) (void)hipFree(dO);
+    if (dP) (void)hipFree(dP);
+    if (dStart) (void)hipFree(dStart);
+    if (dJ) (void)hipFree(dJ);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident dataset
+// ---------------------------------------------------------------------------
+
+struct cgh_dataset {
+    int device;
+    int n, L;
+    size_t n_pad, kw1, kw2;
+    uint32_t *dO, *dP;
+    size_t n_groups, n_members;
+    uint64_t n_rare;
+    uint32_t *dgstart, *dgmem;
+    uint64_t* dgpairs;
+};
+
+extern "C" void cgh_dataset_free(cgh_dataset* ds) {
+    if (!ds) return;
+    (void)hipSetDevice(ds->device);
+    if (ds->dO) (void)hipFree(ds->dO);
+    if (ds->dP) (void)hipFree(ds->dP);
+    if (ds->dgstart) (void)hipFree(ds->dgstart);
+    if (ds->dgmem) (void)hipFree(ds->dgmem);
+    if (ds->dgpairs) (void)hipFree(ds->dgpairs);
+    free(ds);
+}
+
+extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_dataset** out, char* err, int errlen) {
+    cgh_dataset* ds = (cgh_dataset*)calloc(1, sizeof(cgh_dataset));
+    if (!ds) {
+        set_err(err, errlen, "out of host memory");
+        return 1;
+    }
+    *out = NULL;
+    ds->device = device;
+    ds->n = enc->n;
+    ds->L = enc->L;
+    ds->n_pad = enc->n_pad;
+    ds->kw1 = enc->kw1;
+    ds->kw2 = enc->kw2;
+    ds->n_groups = enc->n_groups;
+    ds->n_members = enc->n_members;
+    ds->n_rare = enc->n_rare;
+    if (enc->kw1 % KC || enc->kw2 % KC || enc->kw1 == 0 || enc->n_pad % 256) {
+        set_err(err, errlen, "cgh_dataset_upload: rows must be padded to 256 bits and 256 rows");
+        free(ds);
+        return 1;
+    }
+    HK(hipSetDevice(device));
+    HK(hipMalloc(&ds->dO, enc->n_pad * enc->kw1 * 4));
+    HK(hipMalloc(&ds->dP, enc->n_pad * enc->kw2 * 4));
+    HK(hipMemcpy(ds->dO, enc->O, enc->n_pad * enc->kw1 * 4, hipMemcpyHostToDevice));
+    HK(hipMemcpy(ds->dP, enc->P, enc->n_pad * enc->kw2 * 4, hipMemcpyHostToDevice));
+    if (enc->n_groups) {
+        HK(hipMalloc(&ds->dgstart, (enc->n_groups + 1) * 4));
+        HK(hipMalloc(&ds->dgmem, enc->n_members * 4));
+        HK(hipMalloc(&ds->dgpairs, (enc->n_groups + 1) * 8));
+        HK(hipMemcpy(ds->dgstart, enc->gstart, (enc->n_groups + 1) * 4, hipMemcpyHostToDevice));
+        HK(hipMemcpy(ds->dgmem, enc->gmem, enc->n_members * 4, hipMemcpyHostToDevice));
+        HK(hipMemcpy(ds->dgpairs, enc->gpairs, (enc->n_groups + 1) * 8, hipMemcpyHostToDevice));
+    }
+    *out = ds;
+    return 0;
+fail:
+    cgh_dataset_free(ds);
+    return 1;
+}
+
+extern "C" size_t cgh_stripe_offset(int n, int S, int s) { return stripe_off((size_t)n, (size_t)S, (size_t)s); }
+
+extern "C" size_t cgh_upper_elems(int n, int S) {
+    if (n <= 0 || S <= 0) return 0;
+    const int ns = (n + S - 1) / S;
+    const size_t last = (size_t)(n - (ns - 1) * S);
+    return up(stripe_off((size_t)n, (size_t)S, (size_t)(ns - 1)) + last * (size_t)(n - (ns - 1) * S), 2);
+}
+
+extern "C" int cgh_dataset_tile(const cgh_dataset* ds, int kernel) {
+    (void)kernel;
+    return pick_tile(ds->n);
+}
+
+extern "C" int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, char* err,
+                                 int errlen) {
+    hipStream_t st = (hipStream_t)stream;
+    const int n = ds->n, tile = pick_tile(n);
+    if (S < 8 || S % 8) {
+        set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
+        return 1;
+    }
+    if (kernel != 0) {
+        set_err(err, errlen, "unknown kernel %d", kernel);
+        return 1;
+    }
+    if (((uintptr_t)d_out) & 3) {
+        set_err(err, errlen, "output buffer must be 4-byte aligned");
+        return 1;
+    }
+    HK(hipSetDevice(ds->device));
+    for (int s = 0, r0 = 0; r0 < n; s++, r0 += S) {
+        const int rows = std::min(S, n - r0);
+        HK(launch_stripe(tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, r0, rows,
+                         d_out + stripe_off((size_t)n, (size_t)S, (size_t)s), st));
+    }
+    if (ds->n_rare) {
+        const uint64_t threads = (ds->n_rare + PPT - 1) / PPT;
+        hipLaunchKernelGGL(sub_rare_groups, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, d_out, n, S,
+                           ds->dgstart, ds->dgmem, ds->dgpairs, ds->n_groups, ds->n_rare);
+        HK(hipGetLastError());
+    }
+    return 0;
+fail:
+    return 1;
+}

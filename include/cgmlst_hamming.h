@@ -1,0 +1,152 @@
+/*
+ * cgmlst_hamming.h -- C ABI of the MI355X (gfx950) exact cgMLST allelic
+ * Hamming distance library, libcgmlst_hamming.so.
+ *
+ *   H[i][j] = #loci where both alleles are called (non-zero) and differ
+ *           = P P^T - (O O^T + R)
+ *   P: n x L presence bits, O: n x D one-hot bits of alleles seen >= tau
+ *   times, R[i][j]: loci where i and j share an allele seen 2..tau-1 times.
+ *
+ * Conventions of every entry point: plain C types, borrowed pointers (nothing
+ * is retained after a call returns unless a handle says so), return 0 on
+ * success and 1 on failure with a NUL-terminated message written to err
+ * (truncated to errlen bytes; err may be NULL).  Bit matrices are row-major,
+ * LSB-first within each byte (equivalently within each little-endian 32-bit
+ * word), padding rows and columns zero.
+ */
+#ifndef CGMLST_HAMMING_H
+#define CGMLST_HAMMING_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Reference-compatible boundary: the three symbols the reference host binds. */
+/* ------------------------------------------------------------------------ */
+
+/* Number of visible GPUs, 0 on error.
+ * Replaces tensorcore/rs/cuda/tch.cu:68-71 (declared main.rs:86). */
+int tch_device_count(void);
+
+/* Free device memory in bytes on `device`, 0 on error.  The environment
+ * variable CGH_FREE_BYTES, when set, overrides the figure (capacity planning
+ * can then be exercised without a GPU).
+ * Replaces tensorcore/rs/cuda/tch.cu:73-77 (declared main.rs:87). */
+size_t tch_free_vram(int device);
+
+/* Called once per stripe, in order r0 = 0, S, 2S, ...; stripe is rows x w u16
+ * row-major (column c <-> j = r0 + c, w = n - r0), holding the final H for
+ * c > r (rare pairs already subtracted, no cap, no layout); cells with c <= r
+ * hold 0.  The buffer belongs to the library and is valid during the call.
+ * Same type as tensorcore/rs/cuda/tch.cu:79 / main.rs:84. */
+typedef void (*tch_cb)(void* ctx, int r0, int rows, size_t w, const uint16_t* stripe);
+
+/* Whole upper-triangle pipeline on one GPU.  O: n_pad x K1 bits, P: n_pad x
+ * K2 bits (K1, K2 multiples of 1024, n_pad a multiple of 256, n_pad >= n),
+ * start[n+1]/js[n_rare]: CSR of rare partners j > i per row i, S: stripe rows
+ * (multiple of 8).  *gpu_secs gets the pipeline wall time, *rare_on_gpu 1 if
+ * the CSR fit in device memory (else rare pairs are subtracted on the host).
+ * Replaces tensorcore/rs/cuda/tch.cu:82-170 (declared main.rs:88-92). */
+int tch_run(int device, int n, size_t n_pad,
+            const uint8_t* O, size_t K1, const uint8_t* P, size_t K2,
+            const uint32_t* start, const uint32_t* js, size_t n_rare,
+            int S, tch_cb cb, void* ctx, double* gpu_secs, int* rare_on_gpu,
+            char* err, int errlen);
+
+/* ------------------------------------------------------------------------ */
+/* Host encoder (CPU, multithreaded).                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Dense per-locus ranks of u64 allele ids: 0 stays 0 (missing), id -> 1 +
+ * its index among the sorted distinct non-zero ids of its locus.  Fails with
+ * "locus K has D distinct alleles (limit 65535)" like the reference.
+ * Replaces tensorcore/rs/src/main.rs:135-165 (dense_ranks). */
+int cgh_dense_ranks(const uint64_t* calls, int n, int L, uint16_t* X, char* err, int errlen);
+
+/* Encoded operands of one profile set (host memory, owned by the library
+ * until cgh_encoding_free).  Words are 32-bit, LSB-first. */
+typedef struct cgh_encoding {
+    int n, L;            /* profiles, loci */
+    int tau;             /* frequency threshold used */
+    size_t n_pad;        /* rows allocated in O and P (multiple of 256) */
+    size_t D;            /* one-hot columns (frequent alleles) */
+    size_t kw1, kw2;     /* 32-bit words per row of O and of P */
+    uint32_t* O;         /* n_pad x kw1 words */
+    uint32_t* P;         /* n_pad x kw2 words */
+    size_t n_groups;     /* rare groups: (locus, allele) with 2 <= count < tau */
+    size_t n_members;    /* total members over all groups */
+    uint32_t* gstart;    /* n_groups + 1 offsets into gmem */
+    uint32_t* gmem;      /* member rows, ascending within each group */
+    uint64_t* gpairs;    /* n_groups + 1 prefix sums of c(c-1)/2 */
+    uint64_t n_rare;     /* rare equal pairs = gpairs[n_groups] */
+} cgh_encoding;
+
+/* Encode dense ranks X (n x L u16) at threshold tau >= 1: O/P bits and rare
+ * groups.  Rows of O and P are padded to a multiple of `k_align` bits (256,
+ * 512 or 1024; the reference uses 1024).
+ * Restates tensorcore/rs/src/main.rs:366-403 (column map, bit pack) and
+ * 404-417 (rare groups; the reference expands them into a CSR). */
+int cgh_encode(const uint16_t* X, int n, int L, int tau, int k_align, cgh_encoding* enc,
+               char* err, int errlen);
+
+/* Expand the rare groups of `enc` into the reference's CSR of partners j > i
+ * per row (start[n+1], js[n_rare]; caller-allocated).  Fails with "too many
+ * rare pairs: raise --tau" when n_rare >= 2^32.
+ * Restates tensorcore/rs/src/main.rs:418-444. */
+int cgh_rare_csr(const cgh_encoding* enc, uint32_t* start, uint32_t* js, char* err, int errlen);
+
+void cgh_encoding_free(cgh_encoding* enc);
+
+/* Per-locus allele counts of dense ranks X: counts for ranks 1..max of each
+ * locus, concatenated (zeros included), written to cnt (capacity cap entries);
+ * returns the number of entries needed.  Used by the tau cost model. */
+size_t cgh_allele_counts(const uint16_t* X, int n, int L, uint32_t* cnt, size_t cap);
+
+/* Cost model: tau in {2, 4, ..., 4096} minimising the modelled device time
+ * of the MI355X kernels (MFMA GEMM over D + L bit columns, rare enumeration
+ * over R pairs) subject to operands + rare groups + output stripes fitting in
+ * free_bytes.  Returns 0 when no tau fits ("no tau fits in X GB of free
+ * VRAM").  `log` (may be NULL) receives the per-tau table, one line each.
+ * Restates the selection rule of tensorcore/rs/src/main.rs:169-198 with
+ * rates measured on MI355X instead of the L4's. */
+int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, int S, size_t free_bytes,
+                   void (*log)(void* ctx, const char* line), void* log_ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident dataset (no counterpart in the reference: its pipeline   */
+/* re-uploads operands on every call).                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct cgh_dataset cgh_dataset;
+
+/* Upload O, P and the rare groups of `enc` to `device`. */
+int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_dataset** out, char* err, int errlen);
+void cgh_dataset_free(cgh_dataset* ds);
+
+/* u16 elements of the upper-triangle stripe layout for n profiles and stripe
+ * height S: stripe s (rows r0 = s*S .. r0+rows-1) is rows x (n - r0) u16
+ * row-major, stripes concatenated; rounded up to an even count. */
+size_t cgh_upper_elems(int n, int S);
+
+/* Offset (u16 elements) of stripe s in that layout. */
+size_t cgh_stripe_offset(int n, int S, int s);
+
+/* Exact H of the whole upper triangle into device memory d_out (at least
+ * cgh_upper_elems(n, S) u16), stripe layout, cells with j <= i set to 0.
+ * Enqueued on `stream` (a hipStream_t, NULL = default stream); returns after
+ * enqueueing.  kernel selects the binary product: 0 = int8 MFMA. */
+int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel,
+                      char* err, int errlen);
+
+/* Tile shape (rows = cols) used by cgh_dataset_upper for this dataset. */
+int cgh_dataset_tile(const cgh_dataset* ds, int kernel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
