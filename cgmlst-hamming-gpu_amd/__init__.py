@@ -28,8 +28,10 @@ EXPORTS = (
     "cgh_dense_ranks", "cgh_encode", "cgh_rare_csr", "cgh_encoding_free",
     "cgh_allele_counts", "cgh_choose_tau",
     "cgh_dataset_upload", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
-    "cgh_dataset_upper", "cgh_dataset_tile",
+    "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_tile",
 )
+
+PHASE_PRODUCT, PHASE_RARE = 1, 2
 
 
 class CghError(RuntimeError):
@@ -77,6 +79,7 @@ def load():
         "cgh_upper_elems": (S, [I, I]),
         "cgh_stripe_offset": (S, [I, I, I]),
         "cgh_dataset_upper": (I, [P, I, P, P, I, C.c_char_p, I]),
+        "cgh_dataset_phase": (I, [P, I, P, P, I, I, C.c_char_p, I]),
         "cgh_dataset_tile": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
@@ -219,6 +222,12 @@ class Dataset:
         err = _err()
         _check(load().cgh_dataset_upper(self.handle, S, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), kernel, err,
                                         512), err)
+
+    def phase(self, S: int, out_ptr: int, phases: int, stream_ptr: int = 0, kernel: int = 0):
+        """Enqueue PHASE_PRODUCT and/or PHASE_RARE (cgh_dataset_phase)."""
+        err = _err()
+        _check(load().cgh_dataset_phase(self.handle, S, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), kernel, phases,
+                                        err, 512), err)
 
 
 def upper_to_square(flat: np.ndarray, n: int, S: int) -> np.ndarray:

@@ -38,16 +38,36 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int KC = 8;  // 32-bit words per row staged per pipeline step (256 bits)
 
+// One launch covers `ns` consecutive stripes of S rows starting at row r0.
+// Stripe s (rows r0 + s*S ..) is stored at out + stripe_off(n - r0, S, s) as
+// rows x w u16 with w = n - r0 - s*S, column c <-> j = r0 + s*S + c.  Its
+// tiles are BM x BM, row tiles I = 0..ceil(rows/BM)-1 by column tiles
+// J = 0..ceil(w/BM)-1, numbered stripe by stripe, row-major.  With ns > 1, S
+// is a multiple of BM, so a full stripe has q = S/BM row tiles and nbc - s*q
+// column tiles, and the tile count before stripe s has a closed form.
 struct TileArgs {
     const uint32_t* O;  // n_pad x kw1 words (frequent-allele one-hot bits)
     const uint32_t* P;  // n_pad x kw2 words (presence bits)
     int kw1, kw2;       // words per row, multiples of KC
     int n;              // valid rows
-    int r0, rows;       // first row of this stripe and its row count
-    int w;              // n - r0: output row width, column c <-> j = r0 + c
-    int tiles_n;        // column tiles in this stripe
-    uint16_t* out;      // rows x w u16, row-major
+    int r0;             // first row of the launch
+    int S;              // stripe height
+    int ns;             // stripes in the launch
+    int q;              // row tiles of a full stripe (ns > 1)
+    int nbc;            // column tiles of the first stripe
+    long long T;        // tiles in the launch
+    long long chunk;    // tiles per XCD: consecutive tiles share a row panel in one L2
+    uint16_t* out;      // stripe layout base
 };
+
+// Offset (u16 elements) of stripe s in the concatenated upper-triangle layout.
+__host__ __device__ inline size_t stripe_off(size_t n, size_t S, size_t s) {
+    return S * n * s - S * S * (s * (s - 1) / 2);
+}
+
+__host__ __device__ inline long long tiles_before(long long s, long long q, long long nbc) {
+    return q * (s * nbc - q * (s * (s - 1) / 2));
+}
 
 // 32 bits of one row -> 16 int8 operand bytes of lane half g (0 or 1):
 // byte b of dword t is bit 8b + 4g + t.
@@ -87,14 +107,33 @@ mfma_tile(TileArgs a) {
     __shared__ uint32_t sB[2][KC][BM];
 
     const int tid = threadIdx.x;
-    const int I = blockIdx.x / a.tiles_n, J = blockIdx.x % a.tiles_n;
+    // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+    // contiguous run of tiles so concurrently resident tiles share panels in
+    // that XCD's L2.
+    const long long t = (long long)(blockIdx.x & 7) * a.chunk + (blockIdx.x >> 3);
+    if (t >= a.T) return;
+    int s = 0;
+    if (a.ns > 1) {
+        int lo = 0, hi = a.ns - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tiles_before(mid, a.q, a.nbc) <= t) lo = mid; else hi = mid - 1;
+        }
+        s = lo;
+    }
+    const long long local = t - (s ? tiles_before(s, a.q, a.nbc) : 0);
+    const int tn = a.nbc - s * a.q;
+    const int I = (int)(local / tn), J = (int)(local % tn);
+    const int sr0 = a.r0 + s * a.S;              // first row of the stripe
+    const int rows = min(a.S, a.n - sr0), w = a.n - sr0;
+    uint16_t* __restrict__ out = a.out + stripe_off((size_t)(a.n - a.r0), (size_t)a.S, (size_t)s);
     const int rt0 = I * BM, ct0 = J * BM;  // tile origin, stripe-relative
 
     if (J < I) {  // entirely below the diagonal: every cell has j < i
-        const int rend = min(BM, a.rows - rt0), cend = min(BM, a.w - ct0);
+        const int rend = min(BM, rows - rt0), cend = min(BM, w - ct0);
         for (int e = tid; e < rend * BM; e += NT) {
             const int r = e / BM, c = e % BM;
-            if (c < cend) a.out[(size_t)(rt0 + r) * a.w + ct0 + c] = 0;
+            if (c < cend) out[(size_t)(rt0 + r) * w + ct0 + c] = 0;
         }
         return;
     }
@@ -104,7 +143,7 @@ mfma_tile(TileArgs a) {
     const int l32 = lane & 31, g = lane >> 5;
 
     const int nc1 = a.kw1 / KC, nc = nc1 + a.kw2 / KC;
-    const int growA = a.r0 + rt0, growB = a.r0 + ct0;
+    const int growA = sr0 + rt0, growB = sr0 + ct0;
 
     uint4 stage[LPT];
     auto load_slab = [&](int c) {
@@ -194,11 +233,11 @@ mfma_tile(TileArgs a) {
 #pragma unroll
         for (int j = 0; j < TN; j++) {
             const int c = ct0 + wn0 + j * 32 + l32;
-            if (c >= a.w) continue;
+            if (c >= w) continue;
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int r = rt0 + wm0 + i * 32 + (e >> 2) * 8 + g * 4 + (e & 3);
-                if (r < a.rows) a.out[(size_t)r * a.w + c] = (uint16_t)(c > r ? acc[i][j][e] : 0);
+                if (r < rows) out[(size_t)r * w + c] = (uint16_t)(c > r ? acc[i][j][e] : 0);
             }
         }
 }
@@ -220,11 +259,6 @@ sub_rare_csr(uint16_t* out, int w, int r0, int rows, const uint32_t* start, cons
     const int i = r0 + r;
     const size_t rowoff = (size_t)r * w - r0;
     for (uint32_t p = start[i] + lane; p < start[i + 1]; p += 64) sub_one_u16(out, rowoff + js[p]);
-}
-
-// Offset (u16 elements) of stripe s in the concatenated upper-triangle layout.
-__host__ __device__ inline size_t stripe_off(size_t n, size_t S, size_t s) {
-    return S * n * s - S * S * (s * (s - 1) / 2);
 }
 
 // Rare groups form: pairs (a < b) of each group's ascending member list are
@@ -292,14 +326,27 @@ void set_err(char* err, int errlen, const char* fmt, ...) {
 // wavefronts of 64 x 64) when the 256 grid would leave most CUs idle.
 int pick_tile(int n) { return n >= 8192 ? 256 : 128; }
 
-hipError_t launch_stripe(int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int n, int r0,
-                         int rows, uint16_t* out, hipStream_t st) {
+// Stripes r0, r0+S, ... up to row n in one launch when S is a multiple of the
+// tile (else one launch per stripe), output in the stripe layout at out.
+hipError_t launch_tiles(int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int n, int r0, int S,
+                        int ns, uint16_t* out, hipStream_t st) {
+    if (ns > 1 && S % tile) {
+        for (int s = 0; s < ns; s++) {
+            const hipError_t e = launch_tiles(tile, O, kw1, P, kw2, n, r0 + s * S, S, 1,
+                                              out + stripe_off((size_t)(n - r0), (size_t)S, (size_t)s), st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     TileArgs a;
-    a.O = O; a.P = P; a.kw1 = kw1; a.kw2 = kw2; a.n = n; a.r0 = r0; a.rows = rows; a.w = n - r0;
-    a.out = out;
-    const int tiles_m = (rows + tile - 1) / tile;
-    a.tiles_n = (a.w + tile - 1) / tile;
-    const dim3 grid((unsigned)(tiles_m * a.tiles_n));
+    a.O = O; a.P = P; a.kw1 = kw1; a.kw2 = kw2; a.n = n; a.r0 = r0; a.S = S; a.ns = ns; a.out = out;
+    a.nbc = (n - r0 + tile - 1) / tile;
+    a.q = S / tile;
+    const int last = r0 + (ns - 1) * S;
+    const long long tm_last = (std::min(S, n - last) + tile - 1) / tile;
+    a.T = (ns > 1 ? tiles_before(ns - 1, a.q, a.nbc) : 0) + tm_last * (a.nbc - (long long)(ns - 1) * a.q);
+    a.chunk = (a.T + 7) / 8;
+    const dim3 grid((unsigned)(a.chunk * 8));
     if (tile == 256)
         hipLaunchKernelGGL((mfma_tile<256, 128, 128>), grid, dim3(256), 0, st, a);
     else
@@ -376,7 +423,7 @@ extern "C" int tch_run(int device, int n, size_t n_pad, const uint8_t* O, size_t
         if (r0 < n) {
             const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
             const size_t w = (size_t)(n - r0);
-            HK(launch_stripe(tile, dO, (int)(K1 / 32), dP, (int)(K2 / 32), n, r0, rows, dU[b], st[b]));
+            HK(launch_tiles(tile, dO, (int)(K1 / 32), dP, (int)(K2 / 32), n, r0, S, 1, dU[b], st[b]));
             if (rare_gpu && n_rare) {
                 hipLaunchKernelGGL(sub_rare_csr, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st[b], dU[b],
                                    (int)w, r0, rows, dStart, dJ);
@@ -501,8 +548,8 @@ extern "C" int cgh_dataset_tile(const cgh_dataset* ds, int kernel) {
     return pick_tile(ds->n);
 }
 
-extern "C" int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, char* err,
-                                 int errlen) {
+extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, int phases,
+                                 char* err, int errlen) {
     hipStream_t st = (hipStream_t)stream;
     const int n = ds->n, tile = pick_tile(n);
     if (S < 8 || S % 8) {
@@ -518,12 +565,9 @@ extern "C" int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* 
         return 1;
     }
     HK(hipSetDevice(ds->device));
-    for (int s = 0, r0 = 0; r0 < n; s++, r0 += S) {
-        const int rows = std::min(S, n - r0);
-        HK(launch_stripe(tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, r0, rows,
-                         d_out + stripe_off((size_t)n, (size_t)S, (size_t)s), st));
-    }
-    if (ds->n_rare) {
+    if (phases & CGH_PHASE_PRODUCT)
+        HK(launch_tiles(tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, 0, S, (n + S - 1) / S, d_out, st));
+    if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         const uint64_t threads = (ds->n_rare + PPT - 1) / PPT;
         hipLaunchKernelGGL(sub_rare_groups, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, d_out, n, S,
                            ds->dgstart, ds->dgmem, ds->dgpairs, ds->n_groups, ds->n_rare);
@@ -532,4 +576,9 @@ extern "C" int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* 
     return 0;
 fail:
     return 1;
+}
+
+extern "C" int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, char* err,
+                                 int errlen) {
+    return cgh_dataset_phase(ds, S, d_out, stream, kernel, CGH_PHASE_PRODUCT | CGH_PHASE_RARE, err, errlen);
 }

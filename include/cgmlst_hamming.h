@@ -142,6 +142,14 @@ size_t cgh_stripe_offset(int n, int S, int s);
 int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel,
                       char* err, int errlen);
 
+/* The two phases of cgh_dataset_upper separately (same stream order rules):
+ * CGH_PHASE_PRODUCT writes P P^T - O O^T (every cell of the layout),
+ * CGH_PHASE_RARE then subtracts the rare equal pairs in place. */
+#define CGH_PHASE_PRODUCT 1
+#define CGH_PHASE_RARE 2
+int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, int phases,
+                      char* err, int errlen);
+
 /* Tile shape (rows = cols) used by cgh_dataset_upper for this dataset. */
 int cgh_dataset_tile(const cgh_dataset* ds, int kernel);
 
