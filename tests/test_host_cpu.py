@@ -1,0 +1,125 @@
+"""Host-side pieces on the CPU: the C ABI library loads and exports every
+symbol the header declares, the encoder matches the numpy restatement of
+main.rs:366-444, dense ranks match main.rs:135-165, the tau rule of
+main.rs:169-198 behaves (capacity injected through CGH_FREE_BYTES), and the
+CLI's usage / device errors keep the reference's exit codes.  No kernel is
+launched here."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "cgmlst_hamming.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", text)
+    skip = {"if", "return", "sizeof", "defined"}
+    return sorted({n for n in names if n not in skip and (n.startswith("tch_") or n.startswith("cgh_"))} - {"tch_cb"})
+
+
+def test_library_exports_header(cgh):
+    lib = cgh.load()
+    declared = header_functions()
+    assert set(declared) == set(cgh.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", cgh.LIB_PATH], capture_output=True, text=True).stdout
+    for name in declared:
+        assert re.search(rf"\bT {name}$", out, re.M), name
+
+
+def test_dense_ranks_match_oracle(cgh, workloads):
+    calls = workloads.small_zipf(500, 120, seed=4, id_bits=32)
+    assert np.array_equal(cgh.dense_ranks(calls), oracle.dense_ranks(calls))
+    calls64 = workloads.hashed_ids(calls, bits=64)
+    assert calls64.max() > 2**40
+    X = cgh.dense_ranks(calls64)
+    assert np.array_equal(X, oracle.dense_ranks(calls64))
+    # ranks are ascending in the id within each locus
+    k = 3
+    ids = calls64[:, k]
+    nz = ids != 0
+    order = np.argsort(ids[nz], kind="stable")
+    assert (np.diff(X[nz, k][order].astype(int)) >= 0).all()
+
+
+def test_dense_ranks_limit(cgh):
+    calls = np.arange(1, 70001, dtype=np.uint64).reshape(-1, 1)
+    with pytest.raises(cgh.CghError, match=r"locus 0 has 70000 distinct alleles \(limit 65535\)"):
+        cgh.dense_ranks(calls)
+
+
+@pytest.mark.parametrize("tau", [1, 2, 3, 8, 64, 100000])
+@pytest.mark.parametrize("k_align", [256, 1024])
+def test_encoder_matches_restatement(cgh, tau, k_align):
+    A, _ = oracle.check_make(300, 97, 1, False)
+    X = oracle.dense_ranks(A.astype(np.uint64))
+    enc = cgh.Encoded(X, min(tau, 301), k_align)
+    e = enc.enc
+    D, O, P, groups, partners = oracle.encode_reference(X, tau)
+    assert e.D == D and e.n_pad % 256 == 0 and (e.kw1 * 32) % k_align == 0 and (e.kw2 * 32) % k_align == 0
+    Ob = np.unpackbits(enc.O.view(np.uint8), axis=1, bitorder="little")
+    Pb = np.unpackbits(enc.P.view(np.uint8), axis=1, bitorder="little")
+    assert np.array_equal(Ob[:300, :D], O) and not Ob[:, D:].any() and not Ob[300:].any()
+    assert np.array_equal(Pb[:300, :97], P) and not Pb[:, 97:].any() and not Pb[300:].any()
+    gstart, gmem, gpairs = enc.groups()
+    assert e.n_groups == len(groups)
+    for g, (_, _, mem) in enumerate(groups):
+        assert np.array_equal(gmem[gstart[g]:gstart[g + 1]], mem)
+        c = len(mem)
+        assert gpairs[g + 1] - gpairs[g] == c * (c - 1) // 2
+    start, js = enc.rare_csr()
+    assert e.n_rare == sum(len(p) for p in partners) == js.size
+    for i in range(300):
+        assert sorted(js[start[i]:start[i + 1]].tolist()) == sorted(partners[i])
+
+
+def test_choose_tau_rule(cgh, monkeypatch):
+    A, _ = oracle.check_make(700, 410, 2, True)
+    X = oracle.dense_ranks(A.astype(np.uint64))
+    lines = []
+    tau = cgh.choose_tau(X, 512, 10**12, log=lines.append)
+    assert tau in [2 ** k for k in range(1, 13)] and len(lines) == 12
+    assert cgh.choose_tau(X, 512, 1000) == 0  # nothing fits in 1 kB
+    monkeypatch.setenv("CGH_FREE_BYTES", "123456789")
+    assert cgh.free_vram(0) == 123456789
+
+
+def test_upper_layout_helpers(cgh):
+    n, S = 1000, 192
+    total = 0
+    for s, r0 in enumerate(range(0, n, S)):
+        assert cgh.stripe_offset(n, S, s) == total
+        total += min(S, n - r0) * (n - r0)
+    assert cgh.upper_elems(n, S) == total + (total & 1)
+
+
+def test_cli_usage_errors(cgh):
+    cli = cgh.CLI_PATH
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 2 and "--input <INPUT>" in r.stderr
+    r = subprocess.run([cli, "-i", "x", "-m", "diagonal"], capture_output=True, text=True)
+    assert r.returncode == 2 and "possible values: full, lower-tri, upper-tri" in r.stderr
+    r = subprocess.run([cli, "-i", "x", "--tau", "-3"], capture_output=True, text=True)
+    assert r.returncode == 2
+    r = subprocess.run([cli, "-V"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.startswith("cgmlst-hamming-gpu ")
+    r = subprocess.run([cli, "-h"], capture_output=True, text=True)
+    assert r.returncode == 0 and "--stripe" in r.stdout
+
+
+def test_cli_device_and_separator_errors(cgh):
+    if cgh.device_count() > 0:
+        pytest.skip("a GPU is visible: the device-missing path cannot be reached")
+    cli = cgh.CLI_PATH
+    r = subprocess.run([cli, "-i", "/nonexistent"], capture_output=True, text=True)
+    assert r.returncode == 1 and r.stderr.startswith("error: GPU device 0 not available")
+    r = subprocess.run([cli, "-i", "/nonexistent", "-d", "ab"], capture_output=True, text=True)
+    assert r.returncode == 1 and "input separator must be a single character" in r.stderr
