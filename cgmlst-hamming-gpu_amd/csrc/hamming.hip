@@ -99,12 +99,15 @@ mfma_tile(TileArgs a) {
     constexpr int WAVES_N = BM / WN;
     constexpr int NT = (BM / WM) * WAVES_N * 64;
     constexpr int TM = WM / 32, TN = WN / 32;
-    constexpr int LOADS = 2 * BM * KC / 4;   // 16-byte loads per slab (A and B)
+    constexpr int KG = KC / 4;                  // 4-word groups per slab
+    constexpr int LOADS = 2 * BM * KG;          // 16-byte loads per slab (A and B)
     constexpr int LPT = LOADS / NT;
     static_assert(LOADS % NT == 0, "slab loads must divide evenly");
 
-    __shared__ uint32_t sA[2][KC][BM];
-    __shared__ uint32_t sB[2][KC][BM];
+    // [buffer][group][row] of 4 consecutive words: a wavefront's 16-byte
+    // reads of 32 consecutive rows are contiguous, hence conflict-free.
+    __shared__ uint4 sA[2][KG][BM];
+    __shared__ uint4 sB[2][KG][BM];
 
     const int tid = threadIdx.x;
     // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
@@ -154,12 +157,12 @@ mfma_tile(TileArgs a) {
 #pragma unroll
         for (int u = 0; u < LPT; u++) {
             const int q = tid + u * NT;
-            const int side = q / (BM * KC / 4);
-            const int qq = q % (BM * KC / 4);
-            const int part = qq / BM, row = qq % BM;
+            const int side = q / (BM * KG);
+            const int qq = q % (BM * KG);
+            const int grp = qq / BM, row = qq % BM;
             const int grow = (side ? growB : growA) + row;
             if (grow < a.n)
-                stage[u] = *(const uint4*)(base + (size_t)grow * kw + k0 + part * 4);
+                stage[u] = *(const uint4*)(base + (size_t)grow * kw + k0 + grp * 4);
             else
                 stage[u] = make_uint4(0, 0, 0, 0);
         }
@@ -168,14 +171,10 @@ mfma_tile(TileArgs a) {
 #pragma unroll
         for (int u = 0; u < LPT; u++) {
             const int q = tid + u * NT;
-            const int side = q / (BM * KC / 4);
-            const int qq = q % (BM * KC / 4);
-            const int part = qq / BM, row = qq % BM;
-            uint32_t (*s)[BM] = side ? sB[buf] : sA[buf];
-            s[part * 4 + 0][row] = stage[u].x;
-            s[part * 4 + 1][row] = stage[u].y;
-            s[part * 4 + 2][row] = stage[u].z;
-            s[part * 4 + 3][row] = stage[u].w;
+            const int side = q / (BM * KG);
+            const int qq = q % (BM * KG);
+            const int grp = qq / BM, row = qq % BM;
+            (side ? sB : sA)[buf][grp][row] = stage[u];
         }
     };
 
@@ -191,23 +190,46 @@ mfma_tile(TileArgs a) {
     store_slab(0);
     __syncthreads();
 
-    // O slabs use B = -O (subtracting O O^T), P slabs B = +P.
+    // acc accumulates O O^T - P P^T: O slabs with B = +O, P slabs with
+    // B = -P (the P slabs are the short ones), negated in the epilogue.
+    // Words of group gi+1 are read from LDS while group gi is multiplied.
     auto compute_slab = [&](int buf, auto neg) {
+        uint4 wa[TM], wb[TN];
 #pragma unroll
-        for (int kk = 0; kk < KC; kk++) {
-            v4i av[TM], bv[TN];
+        for (int i = 0; i < TM; i++) wa[i] = sA[buf][0][wm0 + i * 32 + l32];
 #pragma unroll
-            for (int i = 0; i < TM; i++) av[i] = expand_bits(sA[buf][kk][wm0 + i * 32 + l32], g);
+        for (int j = 0; j < TN; j++) wb[j] = sB[buf][0][wn0 + j * 32 + l32];
 #pragma unroll
-            for (int j = 0; j < TN; j++) {
-                const uint32_t wb = sB[buf][kk][wn0 + j * 32 + l32];
-                bv[j] = decltype(neg)::value ? expand_bits_neg(wb, g) : expand_bits(wb, g);
+        for (int gi = 0; gi < KG; gi++) {
+            uint4 na[TM], nb[TN];
+            if (gi + 1 < KG) {
+#pragma unroll
+                for (int i = 0; i < TM; i++) na[i] = sA[buf][gi + 1][wm0 + i * 32 + l32];
+#pragma unroll
+                for (int j = 0; j < TN; j++) nb[j] = sB[buf][gi + 1][wn0 + j * 32 + l32];
             }
 #pragma unroll
-            for (int i = 0; i < TM; i++)
+            for (int kk = 0; kk < 4; kk++) {
+                v4i av[TM], bv[TN];
 #pragma unroll
-                for (int j = 0; j < TN; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[i], bv[j], acc[i][j], 0, 0, 0);
+                for (int i = 0; i < TM; i++) av[i] = expand_bits(((const uint32_t*)&wa[i])[kk], g);
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    const uint32_t x = ((const uint32_t*)&wb[j])[kk];
+                    bv[j] = decltype(neg)::value ? expand_bits_neg(x, g) : expand_bits(x, g);
+                }
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[i], bv[j], acc[i][j], 0, 0, 0);
+            }
+            if (gi + 1 < KG) {
+#pragma unroll
+                for (int i = 0; i < TM; i++) wa[i] = na[i];
+#pragma unroll
+                for (int j = 0; j < TN; j++) wb[j] = nb[j];
+            }
         }
     };
 
@@ -215,19 +237,20 @@ mfma_tile(TileArgs a) {
     for (; c < nc1; c++) {
         const int buf = c & 1;
         load_slab(c + 1);  // nc1 < nc: the P slabs always follow
-        compute_slab(buf, std::true_type{});
+        compute_slab(buf, std::false_type{});
         store_slab(buf ^ 1);
         __syncthreads();
     }
     for (; c < nc; c++) {
         const int buf = c & 1;
         if (c + 1 < nc) load_slab(c + 1);
-        compute_slab(buf, std::false_type{});
+        compute_slab(buf, std::true_type{});
         if (c + 1 < nc) store_slab(buf ^ 1);
         __syncthreads();
     }
 
-    // Epilogue: lane holds column l32 and rows 8*(e/4) + 4*g + (e%4).
+    // Epilogue: lane holds column l32 and rows 8*(e/4) + 4*g + (e%4) of
+    // O O^T - P P^T; the distance is its negation.
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -237,7 +260,7 @@ mfma_tile(TileArgs a) {
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int r = rt0 + wm0 + i * 32 + (e >> 2) * 8 + g * 4 + (e & 3);
-                if (r < rows) out[(size_t)r * w + c] = (uint16_t)(c > r ? acc[i][j][e] : 0);
+                if (r < rows) out[(size_t)r * w + c] = (uint16_t)(c > r ? -acc[i][j][e] : 0);
             }
         }
 }
