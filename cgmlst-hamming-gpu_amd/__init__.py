@@ -46,6 +46,7 @@ class Encoding(C.Structure):
         ("n_groups", C.c_size_t), ("n_members", C.c_size_t),
         ("gstart", C.POINTER(C.c_uint32)), ("gmem", C.POINTER(C.c_uint32)),
         ("gpairs", C.POINTER(C.c_uint64)), ("n_rare", C.c_uint64),
+        ("n_rentries", C.c_size_t), ("rstart", C.POINTER(C.c_uint32)), ("rentry", C.POINTER(C.c_uint64)),
     ]
 
 
@@ -180,6 +181,18 @@ class Encoded:
         e = self.enc
         return (self._arr("gstart", e.n_groups + 1, np.uint32), self._arr("gmem", e.n_members, np.uint32),
                 self._arr("gpairs", e.n_groups + 1, np.uint64))
+
+    def row_index(self):
+        """(rstart[n+1], rentry) -> list of partner arrays per row."""
+        e = self.enc
+        rstart = self._arr("rstart", e.n + 1, np.uint32)
+        rentry = self._arr("rentry", e.n_rentries, np.uint64)
+        gmem = self._arr("gmem", e.n_members, np.uint32)
+        out = []
+        for i in range(e.n):
+            parts = [gmem[int(r & 0xFFFFFFFF):int(r >> 32)] for r in rentry[rstart[i]:rstart[i + 1]]]
+            out.append(np.concatenate(parts) if parts else np.zeros(0, dtype=np.uint32))
+        return out
 
     def rare_csr(self):
         e = self.enc

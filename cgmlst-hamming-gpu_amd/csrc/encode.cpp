@@ -103,6 +103,8 @@ extern "C" void cgh_encoding_free(cgh_encoding* enc) {
     free(enc->gstart);
     free(enc->gmem);
     free(enc->gpairs);
+    free(enc->rstart);
+    free(enc->rentry);
     memset(enc, 0, sizeof *enc);
 }
 
@@ -215,6 +217,39 @@ extern "C" int cgh_encode(const uint16_t* X, int n, int L, int tau, int k_align,
     }
     enc->gpairs[enc->n_groups] = acc;
     enc->n_rare = acc;
+
+    // row index of the same pairs: member p of a group of c gets the range
+    // gmem[gs+p+1, gs+c) of its later partners, listed under row gmem[gs+p]
+    std::vector<std::atomic<uint32_t>> rc((size_t)n + 1);
+    for (auto& x : rc) x.store(0, std::memory_order_relaxed);
+    const long G = (long)enc->n_groups;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long g = 0; g < G; g++) {
+        const uint32_t s0 = enc->gstart[g], s1 = enc->gstart[g + 1];
+        for (uint32_t p = s0; p + 1 < s1; p++) rc[enc->gmem[p]].fetch_add(1, std::memory_order_relaxed);
+    }
+    enc->rstart = (uint32_t*)malloc(((size_t)n + 1) * 4);
+    size_t tot = 0;
+    for (int i = 0; i < n; i++) {
+        enc->rstart[i] = (uint32_t)tot;
+        tot += rc[i].load(std::memory_order_relaxed);
+    }
+    enc->rstart[n] = (uint32_t)tot;
+    enc->n_rentries = tot;
+    enc->rentry = (uint64_t*)malloc(std::max<size_t>(tot, 1) * 8);
+    if (!enc->rstart || !enc->rentry) {
+        cgh_encoding_free(enc);
+        set_err(err, errlen, "out of host memory for the rare row index");
+        return 1;
+    }
+    for (int i = 0; i < n; i++) rc[i].store(enc->rstart[i], std::memory_order_relaxed);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long g = 0; g < G; g++) {
+        const uint32_t s0 = enc->gstart[g], s1 = enc->gstart[g + 1];
+        for (uint32_t p = s0; p + 1 < s1; p++)
+            enc->rentry[rc[enc->gmem[p]].fetch_add(1, std::memory_order_relaxed)] =
+                ((uint64_t)s1 << 32) | (uint64_t)(p + 1);
+    }
     return 0;
 }
 

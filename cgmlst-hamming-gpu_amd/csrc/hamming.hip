@@ -284,43 +284,123 @@ sub_rare_csr(uint16_t* out, int w, int r0, int rows, const uint32_t* start, cons
     for (uint32_t p = start[i] + lane; p < start[i + 1]; p += 64) sub_one_u16(out, rowoff + js[p]);
 }
 
-// Rare groups form: pairs (a < b) of each group's ascending member list are
-// numbered group by group (prefix gpairs); each thread takes PPT consecutive
-// pair numbers, locates its first pair by binary search and walks on.
-constexpr int PPT = 16;
-__global__ void __launch_bounds__(256)
-sub_rare_groups(uint16_t* out, int n, int S, const uint32_t* gstart, const uint32_t* gmem,
-                const uint64_t* gpairs, size_t n_groups, uint64_t n_rare) {
-    const uint64_t p0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * PPT;
-    if (p0 >= n_rare) return;
-    size_t lo = 0, hi = n_groups - 1;  // last group with gpairs[g] <= p0
-    while (lo < hi) {
-        const size_t mid = (lo + hi + 1) / 2;
-        if (gpairs[mid] <= p0) lo = mid; else hi = mid - 1;
-    }
-    size_t grp = lo;
-    uint64_t q = p0 - gpairs[grp];
-    uint32_t gs = gstart[grp], c = gstart[grp + 1] - gs;
-    // pairs before row a: a*(2c - a - 1)/2; find the largest a with that <= q
-    uint32_t pa = (uint32_t)(((2.0 * c - 1.0) - sqrt((2.0 * c - 1.0) * (2.0 * c - 1.0) - 8.0 * (double)q)) / 2.0);
-    if (pa > c - 2) pa = c - 2;
-    while (pa > 0 && (uint64_t)pa * (2 * c - pa - 1) / 2 > q) pa--;
-    while (pa + 1 <= c - 2 && (uint64_t)(pa + 1) * (2 * c - pa - 2) / 2 <= q) pa++;
-    uint32_t pb = pa + 1 + (uint32_t)(q - (uint64_t)pa * (2 * c - pa - 1) / 2);
-    const uint64_t pend = min((uint64_t)n_rare, p0 + PPT);
-    for (uint64_t p = p0; p < pend; p++) {
-        const uint32_t i = gmem[gs + pa], j = gmem[gs + pb];
-        const size_t s = i / (uint32_t)S;
-        const size_t off = stripe_off(n, S, s) + (size_t)(i - s * S) * (size_t)(n - s * S) + (j - s * S);
-        sub_one_u16(out, off);
-        if (++pb == c) {
-            if (++pa == c - 1) {
-                if (++grp >= n_groups) break;
-                gs = gstart[grp];
-                c = gstart[grp + 1] - gs;
-                pa = 0;
+// Rare pairs, row-local: one workgroup per (row i, chunk of RCW columns).
+// The row's cells j in (i, n) of the chunk are staged in LDS with 16-byte
+// loads, every rare partner j of row i is subtracted there with an LDS
+// atomic, and the chunk is written back: HBM sees each output cell read and
+// written once and the scattered updates stay on-chip.  Row i's partners are
+// ranges of the ascending group member lists (rentry); a batch of up to RNT
+// ranges is clipped to the chunk and prefix-summed in LDS, then the block
+// walks the flattened partner list with each lane taking every 64th partner
+// of its wavefront's span, so consecutive lanes read consecutive members and
+// RUN loads are in flight per lane.
+constexpr int RCW = 32768;  // u16 cells per chunk (64 KiB of LDS)
+constexpr int RNT = 512;    // threads per workgroup = ranges per batch
+constexpr int RUN = 8;      // partners per lane per pass
+__global__ void __launch_bounds__(RNT)
+sub_rare_rows(uint16_t* __restrict__ out, int n, int S, int chunks, const uint32_t* __restrict__ rstart,
+              const uint64_t* __restrict__ rentry, const uint32_t* __restrict__ gmem) {
+    __shared__ uint4 cells4[RCW / 8 + 1];
+    __shared__ uint32_t ebeg[RNT];
+    __shared__ uint32_t epre[RNT + 1];
+    __shared__ uint32_t wsum[RNT / 64];
+    uint32_t* cells = reinterpret_cast<uint32_t*>(cells4);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = blockIdx.x / chunks, ch = blockIdx.x % chunks;
+    const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
+    const int j0 = i + 1 + ch * RCW, j1 = min(n, j0 + RCW);
+    if (e0 == e1 || j0 >= n) return;
+    const size_t s = (size_t)(i / S);
+    // u16 index of cell (i, j) is base + j
+    const size_t base = stripe_off((size_t)n, (size_t)S, s) + (size_t)(i - s * S) * (size_t)(n - s * S) - s * S;
+    const size_t g0 = base + j0, g1 = base + j1;   // cells [g0, g1)
+    const size_t a0 = g0 & ~(size_t)7;              // first 16-byte word touched
+    const int nq = (int)((g1 - a0 + 7) >> 3);       // 16-byte words staged
+    const uint4* __restrict__ src = reinterpret_cast<const uint4*>(out + a0);
+    for (int k = tid; k < nq; k += RNT) cells4[k] = src[k];
+
+    for (uint32_t eb = e0; eb < e1; eb += RNT) {
+        const int nb = (int)min((uint32_t)RNT, e1 - eb);
+        uint32_t len = 0;
+        if (tid < nb) {
+            const uint64_t r = rentry[eb + tid];
+            uint32_t b = (uint32_t)r, end = (uint32_t)(r >> 32);
+            if (chunks > 1) {  // clip the ascending range to [j0, j1)
+                uint32_t lo = b, hi = end;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if ((int)gmem[mid] < j0) lo = mid + 1; else hi = mid;
+                }
+                b = lo;
+                hi = end;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if ((int)gmem[mid] < j1) lo = mid + 1; else hi = mid;
+                }
+                end = lo;
             }
-            pb = pa + 1;
+            ebeg[tid] = b;
+            len = end - b;
+        }
+        // block-wide inclusive scan of len -> epre[1..nb], epre[0] = 0
+        uint32_t v = len;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(v, d, 64);
+            if (lane >= d) v += y;
+        }
+        if (lane == 63) wsum[wave] = v;
+        __syncthreads();  // also orders the staging loads before the atomics
+        uint32_t off = 0;
+        for (int w = 0; w < wave; w++) off += wsum[w];
+        epre[tid + 1] = off + v;
+        if (tid == 0) epre[0] = 0;
+        __syncthreads();
+        const uint32_t P = epre[nb];
+        // wavefront w takes spans [w*64*RUN + t*RNT*RUN, +64*RUN)
+        for (uint32_t span = (uint32_t)wave * 64 * RUN; span < P; span += RNT * RUN) {
+            uint32_t q = span + lane;
+            int e = 0;
+            if (q < P) {
+                int lo = 0, hi = nb - 1;  // last e with epre[e] <= q
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (epre[mid] <= q) lo = mid; else hi = mid - 1;
+                }
+                e = lo;
+            }
+            uint32_t idx[RUN];
+            bool ok[RUN];
+#pragma unroll
+            for (int u = 0; u < RUN; u++, q += 64) {
+                ok[u] = q < P;
+                if (ok[u]) {
+                    while (epre[e + 1] <= q) e++;
+                    idx[u] = ebeg[e] + (q - epre[e]);
+                }
+            }
+            uint32_t jj[RUN];
+#pragma unroll
+            for (int u = 0; u < RUN; u++) jj[u] = ok[u] ? gmem[idx[u]] : 0;
+#pragma unroll
+            for (int u = 0; u < RUN; u++)
+                if (ok[u]) {
+                    const size_t o = base + (size_t)jj[u] - a0;  // u16 offset in LDS
+                    atomicSub(&cells[o >> 1], (o & 1) ? 0x10000u : 1u);
+                }
+        }
+        __syncthreads();  // ebeg / epre are rewritten by the next batch
+    }
+
+    uint4* __restrict__ dst = reinterpret_cast<uint4*>(out + a0);
+    for (int k = tid; k < nq; k += RNT) {
+        const size_t lo = a0 + 8 * (size_t)k;  // u16 index of the word's first cell
+        if (lo >= g0 && lo + 8 <= g1) {
+            dst[k] = cells4[k];
+        } else {
+            const uint16_t* c16 = reinterpret_cast<const uint16_t*>(&cells4[k]);
+            for (int h = 0; h < 8; h++)
+                if (lo + h >= g0 && lo + h < g1) out[lo + h] = c16[h];
         }
     }
 }
@@ -501,8 +581,9 @@ struct cgh_dataset {
     uint32_t *dO, *dP;
     size_t n_groups, n_members;
     uint64_t n_rare;
-    uint32_t *dgstart, *dgmem;
-    uint64_t* dgpairs;
+    size_t n_rentries;
+    uint32_t *dgmem, *drstart;
+    uint64_t* drentry;
 };
 
 extern "C" void cgh_dataset_free(cgh_dataset* ds) {
@@ -510,9 +591,9 @@ extern "C" void cgh_dataset_free(cgh_dataset* ds) {
     (void)hipSetDevice(ds->device);
     if (ds->dO) (void)hipFree(ds->dO);
     if (ds->dP) (void)hipFree(ds->dP);
-    if (ds->dgstart) (void)hipFree(ds->dgstart);
     if (ds->dgmem) (void)hipFree(ds->dgmem);
-    if (ds->dgpairs) (void)hipFree(ds->dgpairs);
+    if (ds->drstart) (void)hipFree(ds->drstart);
+    if (ds->drentry) (void)hipFree(ds->drentry);
     free(ds);
 }
 
@@ -532,6 +613,7 @@ extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_datas
     ds->n_groups = enc->n_groups;
     ds->n_members = enc->n_members;
     ds->n_rare = enc->n_rare;
+    ds->n_rentries = enc->n_rentries;
     if (enc->kw1 % KC || enc->kw2 % KC || enc->kw1 == 0 || enc->n_pad % 256) {
         set_err(err, errlen, "cgh_dataset_upload: rows must be padded to 256 bits and 256 rows");
         free(ds);
@@ -543,12 +625,13 @@ extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_datas
     HK(hipMemcpy(ds->dO, enc->O, enc->n_pad * enc->kw1 * 4, hipMemcpyHostToDevice));
     HK(hipMemcpy(ds->dP, enc->P, enc->n_pad * enc->kw2 * 4, hipMemcpyHostToDevice));
     if (enc->n_groups) {
-        HK(hipMalloc(&ds->dgstart, (enc->n_groups + 1) * 4));
         HK(hipMalloc(&ds->dgmem, enc->n_members * 4));
-        HK(hipMalloc(&ds->dgpairs, (enc->n_groups + 1) * 8));
-        HK(hipMemcpy(ds->dgstart, enc->gstart, (enc->n_groups + 1) * 4, hipMemcpyHostToDevice));
+        HK(hipMalloc(&ds->drstart, ((size_t)enc->n + 1) * 4));
+        HK(hipMalloc(&ds->drentry, std::max<size_t>(enc->n_rentries, 1) * 8));
         HK(hipMemcpy(ds->dgmem, enc->gmem, enc->n_members * 4, hipMemcpyHostToDevice));
-        HK(hipMemcpy(ds->dgpairs, enc->gpairs, (enc->n_groups + 1) * 8, hipMemcpyHostToDevice));
+        HK(hipMemcpy(ds->drstart, enc->rstart, ((size_t)enc->n + 1) * 4, hipMemcpyHostToDevice));
+        if (enc->n_rentries)
+            HK(hipMemcpy(ds->drentry, enc->rentry, enc->n_rentries * 8, hipMemcpyHostToDevice));
     }
     *out = ds;
     return 0;
@@ -563,7 +646,7 @@ extern "C" size_t cgh_upper_elems(int n, int S) {
     if (n <= 0 || S <= 0) return 0;
     const int ns = (n + S - 1) / S;
     const size_t last = (size_t)(n - (ns - 1) * S);
-    return up(stripe_off((size_t)n, (size_t)S, (size_t)(ns - 1)) + last * (size_t)(n - (ns - 1) * S), 2);
+    return up(stripe_off((size_t)n, (size_t)S, (size_t)(ns - 1)) + last * (size_t)(n - (ns - 1) * S), 8);
 }
 
 extern "C" int cgh_dataset_tile(const cgh_dataset* ds, int kernel) {
@@ -591,9 +674,9 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
     if (phases & CGH_PHASE_PRODUCT)
         HK(launch_tiles(tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, 0, S, (n + S - 1) / S, d_out, st));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
-        const uint64_t threads = (ds->n_rare + PPT - 1) / PPT;
-        hipLaunchKernelGGL(sub_rare_groups, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, d_out, n, S,
-                           ds->dgstart, ds->dgmem, ds->dgpairs, ds->n_groups, ds->n_rare);
+        const int chunks = (n - 1 + RCW - 1) / RCW;
+        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)n * chunks)), dim3(RNT), 0, st, d_out, n, S, chunks,
+                           ds->drstart, ds->drentry, ds->dgmem);
         HK(hipGetLastError());
     }
     return 0;

@@ -83,6 +83,11 @@ typedef struct cgh_encoding {
     uint32_t* gmem;      /* member rows, ascending within each group */
     uint64_t* gpairs;    /* n_groups + 1 prefix sums of c(c-1)/2 */
     uint64_t n_rare;     /* rare equal pairs = gpairs[n_groups] */
+    /* The same pairs indexed by their first row: row i's rare calls that
+     * have later group members, as ranges of gmem (the members j > i). */
+    size_t n_rentries;   /* entries over all rows */
+    uint32_t* rstart;    /* n + 1 offsets into rentry */
+    uint64_t* rentry;    /* (end << 32) | begin, a range of gmem */
 } cgh_encoding;
 
 /* Encode dense ranks X (n x L u16) at threshold tau >= 1: O/P bits and rare
@@ -129,7 +134,8 @@ void cgh_dataset_free(cgh_dataset* ds);
 
 /* u16 elements of the upper-triangle stripe layout for n profiles and stripe
  * height S: stripe s (rows r0 = s*S .. r0+rows-1) is rows x (n - r0) u16
- * row-major, stripes concatenated; rounded up to an even count. */
+ * row-major, stripes concatenated; rounded up to a multiple of 8 (whole
+ * 16-byte words, which the rare-pair pass stages). */
 size_t cgh_upper_elems(int n, int S);
 
 /* Offset (u16 elements) of stripe s in that layout. */

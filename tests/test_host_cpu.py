@@ -77,8 +77,10 @@ def test_encoder_matches_restatement(cgh, tau, k_align):
         assert gpairs[g + 1] - gpairs[g] == c * (c - 1) // 2
     start, js = enc.rare_csr()
     assert e.n_rare == sum(len(p) for p in partners) == js.size
+    rows = enc.row_index()
     for i in range(300):
         assert sorted(js[start[i]:start[i + 1]].tolist()) == sorted(partners[i])
+        assert sorted(rows[i].tolist()) == sorted(partners[i])
 
 
 def test_choose_tau_rule(cgh, monkeypatch):
@@ -98,7 +100,7 @@ def test_upper_layout_helpers(cgh):
     for s, r0 in enumerate(range(0, n, S)):
         assert cgh.stripe_offset(n, S, s) == total
         total += min(S, n - r0) * (n - r0)
-    assert cgh.upper_elems(n, S) == total + (total & 1)
+    assert cgh.upper_elems(n, S) == (total + 7) // 8 * 8
 
 
 def test_cli_usage_errors(cgh):
