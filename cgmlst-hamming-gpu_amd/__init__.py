@@ -28,7 +28,7 @@ EXPORTS = (
     "cgh_dense_ranks", "cgh_encode", "cgh_rare_csr", "cgh_encoding_free",
     "cgh_allele_counts", "cgh_choose_tau",
     "cgh_dataset_upload", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
-    "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_tile",
+    "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream", "cgh_dataset_tile",
 )
 
 PHASE_PRODUCT, PHASE_RARE = 1, 2
@@ -74,13 +74,14 @@ def load():
         "cgh_rare_csr": (I, [C.POINTER(Encoding), P, P, C.c_char_p, I]),
         "cgh_encoding_free": (None, [C.POINTER(Encoding)]),
         "cgh_allele_counts": (S, [P, I, I, P, S]),
-        "cgh_choose_tau": (I, [P, S, I, I, I, S, LOG_CB, P]),
+        "cgh_choose_tau": (I, [P, S, I, I, S, S, LOG_CB, P]),
         "cgh_dataset_upload": (I, [I, C.POINTER(Encoding), C.POINTER(P), C.c_char_p, I]),
         "cgh_dataset_free": (None, [P]),
         "cgh_upper_elems": (S, [I, I]),
         "cgh_stripe_offset": (S, [I, I, I]),
         "cgh_dataset_upper": (I, [P, I, P, P, I, C.c_char_p, I]),
         "cgh_dataset_phase": (I, [P, I, P, P, I, I, C.c_char_p, I]),
+        "cgh_dataset_stream": (I, [P, I, TCH_CB, P, C.POINTER(C.c_double), C.c_char_p, I]),
         "cgh_dataset_tile": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
@@ -133,13 +134,14 @@ def allele_counts(X: np.ndarray) -> np.ndarray:
     return cnt[:need]
 
 
-def choose_tau(X: np.ndarray, stripe: int, free_bytes: int, log=None) -> int:
-    """Auto tau (main.rs:169-198 rule, MI355X rates); 0 if none fits."""
+def choose_tau(X: np.ndarray, out_bytes: int, free_bytes: int, log=None) -> int:
+    """Auto tau (main.rs:169-198 rule, MI355X rates) for a run that
+    allocates out_bytes of output buffers; 0 if none fits."""
     n, L = X.shape
     cnt = allele_counts(X)
     lines = []
     cb = LOG_CB(lambda ctx, line: lines.append(line.decode()))
-    tau = load().cgh_choose_tau(_ptr(cnt), cnt.size, n, L, stripe, free_bytes, cb, None)
+    tau = load().cgh_choose_tau(_ptr(cnt), cnt.size, n, L, out_bytes, free_bytes, cb, None)
     if log is not None:
         for ln in lines:
             log(ln)
@@ -226,6 +228,25 @@ class Dataset:
             self.handle = C.c_void_p()
 
     __del__ = close
+
+    def stream(self, S: int):
+        """cgh_dataset_stream into a square symmetric H (host)."""
+        n = self.n
+        H = np.zeros((n, n), dtype=np.uint16)
+        seen = []
+
+        def cb(ctx, r0, rows, w, stripe):
+            H[r0:r0 + rows, r0:] = np.ctypeslib.as_array(stripe, shape=(rows * w,)).reshape(rows, w)
+            seen.append((r0, rows, w))
+
+        ccb = TCH_CB(cb)
+        secs = C.c_double()
+        err = _err()
+        _check(load().cgh_dataset_stream(self.handle, S, ccb, None, C.byref(secs), err, 512), err)
+        iu = np.triu_indices(n, 1)
+        H.T[iu] = H[iu]
+        np.fill_diagonal(H, 0)
+        return H, secs.value, seen
 
     def tile(self, kernel: int = 0) -> int:
         return load().cgh_dataset_tile(self.handle, kernel)

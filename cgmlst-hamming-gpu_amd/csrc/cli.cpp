@@ -562,7 +562,9 @@ int main(int argc, char** argv) {
             std::vector<uint32_t> cnt(cgh_allele_counts(X.data(), (int)n, (int)l, NULL, 0));
             cgh_allele_counts(X.data(), (int)n, (int)l, cnt.data(), cnt.size());
             const size_t freeb = tch_free_vram((int)args.device);
-            tau = (size_t)cgh_choose_tau(cnt.data(), cnt.size(), (int)n, (int)l, (int)stripe, freeb, log_line,
+            const size_t n_pad = (n + 255) / 256 * 256;
+            const size_t out_bytes = 2 * ((stripe * n_pad + 7) / 8 * 8) * 2;  // two stripe buffers
+            tau = (size_t)cgh_choose_tau(cnt.data(), cnt.size(), (int)n, (int)l, out_bytes, freeb, log_line,
                                          &args.silent);
             if (tau == 0) {
                 snprintf(msg, sizeof msg, "no tau fits in %.2f GB of free VRAM", (double)freeb / 1e9);
@@ -571,10 +573,7 @@ int main(int argc, char** argv) {
         }
         const int tau_i = (int)std::min<size_t>(tau, (size_t)n + 1);  // counts never exceed n
         cgh_encoding enc;
-        if (cgh_encode(X.data(), (int)n, (int)l, tau_i, 1024, &enc, err, sizeof err)) die(err);
-        if (enc.n_rare >= (1ull << 32)) die("too many rare pairs: raise --tau");
-        std::vector<uint32_t> start(n + 1), js(std::max<uint64_t>(enc.n_rare, 1));
-        if (cgh_rare_csr(&enc, start.data(), js.data(), err, sizeof err)) die(err);
+        if (cgh_encode(X.data(), (int)n, (int)l, tau_i, 256, &enc, err, sizeof err)) die(err);
         snprintf(msg, sizeof msg, "Encoded in %.2fs: tau=%zu, one-hot cols=%zu, rare equal pairs=%llu", now() - t, tau,
                  enc.D, (unsigned long long)enc.n_rare);
         log(msg);
@@ -582,15 +581,14 @@ int main(int argc, char** argv) {
         // ---- GPU (main.rs:447-465) ----
         t = now();
         double gpu_secs = 0;
-        int rare_gpu = 0;
-        const int rc = tch_run((int)args.device, (int)n, enc.n_pad, (const uint8_t*)enc.O, enc.kw1 * 32,
-                               (const uint8_t*)enc.P, enc.kw2 * 32, start.data(), js.data(), (size_t)enc.n_rare,
-                               (int)stripe, on_stripe, &up, &gpu_secs, &rare_gpu, err, sizeof err);
+        cgh_dataset* ds = NULL;
+        if (cgh_dataset_upload((int)args.device, &enc, &ds, err, sizeof err)) die(std::string("GPU: ") + err);
         cgh_encoding_free(&enc);
+        const int rc = cgh_dataset_stream(ds, (int)stripe, on_stripe, &up, &gpu_secs, err, sizeof err);
+        cgh_dataset_free(ds);
         if (rc) die(std::string("GPU: ") + err);
-        snprintf(msg, sizeof msg,
-                 "Computed %zu x %zu matrix on MFMA in %.2fs (GPU pipeline %.2fs, rare pairs on %s)", n, n,
-                 now() - t, gpu_secs, rare_gpu ? "GPU" : "host");
+        snprintf(msg, sizeof msg, "Computed %zu x %zu matrix on MFMA in %.2fs (GPU pipeline %.2fs, rare pairs on GPU)",
+                 n, n, now() - t, gpu_secs);
         log(msg);
     } else {
         for (size_t i = 0; i < n; i++) up.rows[i].assign(n - 1 - i, 0);

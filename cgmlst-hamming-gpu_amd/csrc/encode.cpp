@@ -287,35 +287,45 @@ extern "C" int cgh_rare_csr(const cgh_encoding* enc, uint32_t* start, uint32_t* 
     return 0;
 }
 
-// Rates of the MI355X kernels used by the cost model (measured with bench.py
-// on the 63,005 x 1,142 long-tail workload): binary GEMM in pair-bits per
-// second over the triangle, rare enumeration in pairs per second.
-static const double GEMM_RATE = 1.2e15;
-static const double RARE_RATE = 1.0e10;
+// Device-time model of one upper-triangle pass on MI355X, fitted to bench.py
+// runs on the 63,005 x 1,142 long-tail workload at tau = 256 .. 2048:
+//   MFMA binary product   n^2/2 * (K1 + K2) pair-bits at GEMM_RATE
+//   rare-pair pass        each output cell staged through LDS once
+//                         (read + write, STAGE_BW) plus R pairs at RARE_RATE
+static const double GEMM_RATE = 1.5e15;   // pair-bits per second
+static const double STAGE_BW = 2.0e12;    // bytes per second, rare-pass staging
+static const double RARE_RATE = 5.4e11;   // rare pairs per second
 
-extern "C" int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, int S, size_t free_bytes,
+extern "C" int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, size_t out_bytes, size_t free_bytes,
                               void (*log)(void* ctx, const char* line), void* log_ctx) {
     const double n_pad = (double)up((size_t)std::max(n, 1), 256);
-    const double k2 = (double)up((size_t)std::max(L, 1), 1024);
-    const double bufs = (double)S * n_pad * 2.0 * 2.0 + (double)(64u << 20);
+    const double k2 = (double)up((size_t)std::max(L, 1), 256);
+    const double pairs = (double)n * (double)(n - 1) / 2.0;
     int best = 0;
     double best_t = 1e300;
     for (int tau = 2; tau <= 4096; tau *= 2) {
         size_t d = 0;
-        double r = 0;
+        double r = 0, members = 0;
         for (size_t t = 0; t < ncnt; t++) {
             const uint32_t x = cnt[t];
             if (!x) continue;
-            if (x >= (uint32_t)tau) d++;
-            else r += (double)x * ((double)x - 1.0) / 2.0;
+            if (x >= (uint32_t)tau) {
+                d++;
+            } else if (x >= 2) {
+                r += (double)x * ((double)x - 1.0) / 2.0;
+                members += x;
+            }
         }
-        const double k1 = (double)up(std::max<size_t>(d, 1), 1024);
-        const double mem = n_pad * (k1 + k2) / 8.0 + 4.0 * (r + n + 1.0) + bufs;
-        const double t = (double)n * (double)n / 2.0 * (k1 + k2) / GEMM_RATE + r / RARE_RATE;
-        const bool fits = mem <= (double)free_bytes && r < 4.0e9;
+        const double k1 = (double)up(std::max<size_t>(d, 1), 256);
+        // operands + group members (4 B) + row index (8 B per member at most)
+        // + row offsets + output buffers + slack
+        const double mem = n_pad * (k1 + k2) / 8.0 + 12.0 * members + 4.0 * (n + 1.0) + (double)out_bytes +
+                           (double)(64u << 20);
+        const double t = pairs * (k1 + k2) / GEMM_RATE + (r > 0 ? pairs * 4.0 / STAGE_BW + r / RARE_RATE : 0.0);
+        const bool fits = mem <= (double)free_bytes && members < 4.0e9;
         if (log) {
             char line[256];
-            snprintf(line, sizeof line, "  tau=%5d  one-hot=%8zu  rare=%14.0f  est %.3fs  vram %.2f GB%s", tau, d, r, t,
+            snprintf(line, sizeof line, "  tau=%5d  one-hot=%8zu  rare=%14.0f  est %.4fs  vram %.2f GB%s", tau, d, r, t,
                      mem / 1e9, fits ? "" : "  (does not fit)");
             log(log_ctx, line);
         }

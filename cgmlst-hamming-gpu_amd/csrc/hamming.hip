@@ -298,7 +298,7 @@ constexpr int RCW = 32768;  // u16 cells per chunk (64 KiB of LDS)
 constexpr int RNT = 512;    // threads per workgroup = ranges per batch
 constexpr int RUN = 8;      // partners per lane per pass
 __global__ void __launch_bounds__(RNT)
-sub_rare_rows(uint16_t* __restrict__ out, int n, int S, int chunks, const uint32_t* __restrict__ rstart,
+sub_rare_rows(uint16_t* __restrict__ out, int n, int r0, int S, int chunks, const uint32_t* __restrict__ rstart,
               const uint64_t* __restrict__ rentry, const uint32_t* __restrict__ gmem) {
     __shared__ uint4 cells4[RCW / 8 + 1];
     __shared__ uint32_t ebeg[RNT];
@@ -306,13 +306,14 @@ sub_rare_rows(uint16_t* __restrict__ out, int n, int S, int chunks, const uint32
     __shared__ uint32_t wsum[RNT / 64];
     uint32_t* cells = reinterpret_cast<uint32_t*>(cells4);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i = blockIdx.x / chunks, ch = blockIdx.x % chunks;
+    const int i = r0 + (int)(blockIdx.x / chunks), ch = blockIdx.x % chunks;
     const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
     const int j0 = i + 1 + ch * RCW, j1 = min(n, j0 + RCW);
     if (e0 == e1 || j0 >= n) return;
-    const size_t s = (size_t)(i / S);
+    // out holds the stripe layout of rows r0.. (stripe s = rows r0+s*S ..);
     // u16 index of cell (i, j) is base + j
-    const size_t base = stripe_off((size_t)n, (size_t)S, s) + (size_t)(i - s * S) * (size_t)(n - s * S) - s * S;
+    const size_t s = (size_t)((i - r0) / S), sr0 = (size_t)r0 + s * S;
+    const size_t base = stripe_off((size_t)(n - r0), (size_t)S, s) + ((size_t)i - sr0) * ((size_t)n - sr0) - sr0;
     const size_t g0 = base + j0, g1 = base + j1;   // cells [g0, g1)
     const size_t a0 = g0 & ~(size_t)7;              // first 16-byte word touched
     const int nq = (int)((g1 - a0 + 7) >> 3);       // 16-byte words staged
@@ -675,7 +676,7 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
         HK(launch_tiles(tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, 0, S, (n + S - 1) / S, d_out, st));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         const int chunks = (n - 1 + RCW - 1) / RCW;
-        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)n * chunks)), dim3(RNT), 0, st, d_out, n, S, chunks,
+        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)n * chunks)), dim3(RNT), 0, st, d_out, n, 0, S, chunks,
                            ds->drstart, ds->drentry, ds->dgmem);
         HK(hipGetLastError());
     }
@@ -687,4 +688,70 @@ fail:
 extern "C" int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, char* err,
                                  int errlen) {
     return cgh_dataset_phase(ds, S, d_out, stream, kernel, CGH_PHASE_PRODUCT | CGH_PHASE_RARE, err, errlen);
+}
+
+// Stripes streamed to the host: the reference pipeline's shape (tch.cu:82-170,
+// two device buffers, two streams, the host consumes stripe s while stripe
+// s+1 is computed) driven by the device-resident dataset, so the rare pairs
+// never have to be expanded into a per-pair list.
+extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, char* err,
+                                  int errlen) {
+    uint16_t *dU[2] = {NULL, NULL}, *hU[2] = {NULL, NULL};
+    hipStream_t st[2] = {NULL, NULL};
+    hipEvent_t done[2] = {NULL, NULL}, t0 = NULL, t1 = NULL;
+    int rc = 1, prev = -1;
+    float ms = 0.f;
+    const int n = ds->n, tile = pick_tile(n);
+    const size_t stripe_elems = up((size_t)S * ds->n_pad, 8);
+    if (S < 8 || S % 8) {
+        set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
+        return 1;
+    }
+    HK(hipSetDevice(ds->device));
+    for (int b = 0; b < 2; b++) {
+        HK(hipMalloc(&dU[b], stripe_elems * 2));
+        HK(hipHostMalloc(&hU[b], stripe_elems * 2, hipHostMallocDefault));
+        HK(hipStreamCreateWithFlags(&st[b], hipStreamNonBlocking));
+        HK(hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
+    }
+    HK(hipEventCreate(&t0));
+    HK(hipEventCreate(&t1));
+    HK(hipEventRecord(t0, st[0]));
+    for (int r0 = 0;; r0 += S) {
+        if (r0 < n) {
+            const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
+            const size_t w = (size_t)(n - r0);
+            HK(launch_tiles(tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, r0, S, 1, dU[b], st[b]));
+            if (ds->n_rare && n - r0 > 1) {
+                const int chunks = (n - r0 - 1 + RCW - 1) / RCW;
+                hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)rows * chunks)), dim3(RNT), 0, st[b], dU[b],
+                                   n, r0, S, chunks, ds->drstart, ds->drentry, ds->dgmem);
+                HK(hipGetLastError());
+            }
+            HK(hipMemcpyAsync(hU[b], dU[b], (size_t)rows * w * 2, hipMemcpyDeviceToHost, st[b]));
+            HK(hipEventRecord(done[b], st[b]));
+        }
+        if (prev >= 0) {  // consume the previous stripe while this one runs
+            const int rows = std::min(S, n - prev), b = (prev / S) & 1;
+            HK(hipEventSynchronize(done[b]));
+            cb(ctx, prev, rows, (size_t)(n - prev), hU[b]);
+        }
+        if (r0 >= n) break;
+        prev = r0;
+    }
+    HK(hipEventRecord(t1, st[0]));
+    HK(hipEventSynchronize(t1));
+    HK(hipEventElapsedTime(&ms, t0, t1));
+    if (gpu_secs) *gpu_secs = ms / 1e3;
+    rc = 0;
+fail:
+    for (int b = 0; b < 2; b++) {
+        if (dU[b]) (void)hipFree(dU[b]);
+        if (hU[b]) (void)hipHostFree(hU[b]);
+        if (st[b]) (void)hipStreamDestroy(st[b]);
+        if (done[b]) (void)hipEventDestroy(done[b]);
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    return rc;
 }

@@ -112,13 +112,13 @@ void cgh_encoding_free(cgh_encoding* enc);
 size_t cgh_allele_counts(const uint16_t* X, int n, int L, uint32_t* cnt, size_t cap);
 
 /* Cost model: tau in {2, 4, ..., 4096} minimising the modelled device time
- * of the MI355X kernels (MFMA GEMM over D + L bit columns, rare enumeration
- * over R pairs) subject to operands + rare groups + output stripes fitting in
- * free_bytes.  Returns 0 when no tau fits ("no tau fits in X GB of free
- * VRAM").  `log` (may be NULL) receives the per-tau table, one line each.
+ * of the MI355X kernels (MFMA product over D + L bit columns, row-local rare
+ * pass over R pairs) subject to operands + rare index + out_bytes of output
+ * buffers fitting in free_bytes.  Returns 0 when no tau fits ("no tau fits
+ * in X GB of free VRAM").  `log` (may be NULL) receives the per-tau table.
  * Restates the selection rule of tensorcore/rs/src/main.rs:169-198 with
  * rates measured on MI355X instead of the L4's. */
-int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, int S, size_t free_bytes,
+int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, size_t out_bytes, size_t free_bytes,
                    void (*log)(void* ctx, const char* line), void* log_ctx);
 
 /* ------------------------------------------------------------------------ */
@@ -155,6 +155,14 @@ int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int
 #define CGH_PHASE_RARE 2
 int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, int phases,
                       char* err, int errlen);
+
+/* Exact H streamed to the host in stripes of S rows (multiple of 8): the
+ * callback contract of tch_run (stripe order, rows x (n - r0) u16 final H
+ * for c > r, 0 for c <= r, library-owned buffer valid during the call), with
+ * the rare pairs taken from the dataset's groups instead of a per-pair CSR.
+ * *gpu_secs (may be NULL) gets the pipeline wall time.  The CLI's path;
+ * no counterpart in the reference beyond tch_run's shape. */
+int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, char* err, int errlen);
 
 /* Tile shape (rows = cols) used by cgh_dataset_upper for this dataset. */
 int cgh_dataset_tile(const cgh_dataset* ds, int kernel);

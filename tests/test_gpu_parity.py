@@ -101,3 +101,46 @@ def test_tch_run(cgh, gpu, monkeypatch, force_host):
     assert rare_gpu == (not force_host)
     assert [s[0] for s in seen] == list(range(0, 700, 64))
     assert np.array_equal(H, oracle.check_ref(A).astype(np.uint16))
+
+
+@pytest.mark.parametrize("S,tau", [(64, 8), (520, 64), (4096, 1024), (8, 2)])
+def test_dataset_stream(cgh, gpu, workloads, S, tau):
+    """cgh_dataset_stream (the CLI's path): stripes in order, every cell."""
+    calls = workloads.small_zipf(1500, 700, seed=9)
+    X = cgh.dense_ranks(calls)
+    enc = cgh.Encoded(X, tau, 256)
+    ds = cgh.Dataset(enc)
+    print(f"dataset_stream S={S} tau={tau}", flush=True)
+    H, secs, seen = ds.stream(S)
+    ds.close()
+    assert [s[0] for s in seen] == list(range(0, 1500, S))
+    assert np.array_equal(H, oracle.brute_full(X))
+
+
+def test_rare_pass_many_chunks(cgh, gpu):
+    """Rows longer than one LDS chunk (n > 32,768): chunk clipping of the
+    partner ranges, checked on a row sample against the oracle."""
+    rng = np.random.default_rng(12)
+    n, L = 40000, 24
+    X = rng.integers(1, 300, size=(n, L)).astype(np.uint16)   # ~133 copies per allele: rare at tau=4096
+    X[rng.random((n, L)) < 0.05] = 0
+    S = 4096
+    enc = cgh.Encoded(X, 4096, 256)
+    assert enc.enc.n_rare > 5 * 10**7
+    ds = cgh.Dataset(enc)
+    out = gpu.zeros(cgh.upper_elems(n, S), dtype=gpu.int16, device="cuda")
+    print("rare many chunks", flush=True)
+    ds.upper(S, out.data_ptr(), gpu.cuda.current_stream().cuda_stream)
+    gpu.cuda.synchronize()
+    flat = out.cpu().numpy().view(np.uint16)
+    ds.close()
+    rows = [0, 1, 2, 4095, 4096, 8191, 20000, 39998]
+    ref = oracle.brute_rows(X, 0, 1)  # warm the library
+    for i in rows:
+        s = i // S
+        r0 = s * S
+        w = n - r0
+        off = cgh.stripe_offset(n, S, s) + (i - r0) * w
+        got = flat[off + (i + 1 - r0): off + w]
+        ref = oracle.brute_rows(X, i, i + 1)[0, i + 1:]
+        assert np.array_equal(got, ref), i

@@ -87,9 +87,9 @@ def test_choose_tau_rule(cgh, monkeypatch):
     A, _ = oracle.check_make(700, 410, 2, True)
     X = oracle.dense_ranks(A.astype(np.uint64))
     lines = []
-    tau = cgh.choose_tau(X, 512, 10**12, log=lines.append)
+    tau = cgh.choose_tau(X, 2 * 512 * 768 * 2, 10**12, log=lines.append)
     assert tau in [2 ** k for k in range(1, 13)] and len(lines) == 12
-    assert cgh.choose_tau(X, 512, 1000) == 0  # nothing fits in 1 kB
+    assert cgh.choose_tau(X, 2 * 512 * 768 * 2, 1000) == 0  # nothing fits in 1 kB
     monkeypatch.setenv("CGH_FREE_BYTES", "123456789")
     assert cgh.free_vram(0) == 123456789
 
