@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--tau", type=int, default=0, help="0 = cost model")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the distance path runs on a single device "
@@ -90,6 +92,14 @@ def main():
     import torch
 
     torch.zeros(1, device="cuda")  # torch's HIP runtime first; the library shares it
+    if args.copy_calibration:
+        src = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 1 << 30, "copies": 3}), flush=True)
+        return
     cgh = load_package()
     import cgmlst_hamming_gpu_amd.workloads as wl
 
