@@ -150,6 +150,15 @@ def main():
     achieved = ops / (prod_ms * 1e-3) / 1e12
     log(f"step {ms_step:.2f} ms (product {prod_ms:.2f} ms, rare {rare_ms:.2f} ms)")
 
+    # HBM bytes per launch of the product kernel, from the committed counter
+    # passes (profiles/hbm_traffic_*.json) when they cover this configuration
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_long_tail_63005x1142.json")
+    if os.path.exists(tpath):
+        t_rec = json.load(open(tpath))
+        if (t_rec["n"], t_rec["L"], t_rec["tau"], t_rec["stripe"]) == (N, L, tau, S):
+            traffic = t_rec["kernels"]["mfma_tile"]["traffic_bytes"]
+
     flat = out.cpu().numpy().view(np.uint16)
     cpu = None if args.no_cpu_baseline else cpu_baseline(X, flat, cgh)
     if cpu is not None:
@@ -172,7 +181,7 @@ def main():
         "config": {"workload": "long_tail", "n": N, "L": L, "tau": tau, "stripe": S, "layout": "upper-tri u16",
                    "one_hot_cols": D, "rare_pairs": R, "tile": ds.tile()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
-                     "frac": achieved / PEAK_INT8_TOPS, "traffic": None,
+                     "frac": achieved / PEAK_INT8_TOPS, "traffic": traffic,
                      "kernel": "mfma_tile (binary product + u16 epilogue, one launch per step)",
                      "work": "2*(D+L) int8 ops per pair, pairs = n(n-1)/2"},
         "phases_ms": {"product": prod_ms, "rare": rare_ms},
