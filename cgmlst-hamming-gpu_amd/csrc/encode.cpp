@@ -292,9 +292,9 @@ extern "C" int cgh_rare_csr(const cgh_encoding* enc, uint32_t* start, uint32_t* 
 //   MFMA binary product   n^2/2 * (K1 + K2) pair-bits at GEMM_RATE
 //   rare-pair pass        each output cell staged through LDS once
 //                         (read + write, STAGE_BW) plus R pairs at RARE_RATE
-static const double GEMM_RATE = 1.5e15;   // pair-bits per second
-static const double STAGE_BW = 2.0e12;    // bytes per second, rare-pass staging
-static const double RARE_RATE = 5.4e11;   // rare pairs per second
+static const double GEMM_RATE = 2.25e15;  // pair-bits per second (fp4 MFMA kernel)
+static const double STAGE_BW = 2.6e12;    // bytes per second, rare-pass staging
+static const double RARE_RATE = 9.0e11;   // rare pairs per second
 
 extern "C" int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, size_t out_bytes, size_t free_bytes,
                               void (*log)(void* ctx, const char* line), void* log_ctx) {

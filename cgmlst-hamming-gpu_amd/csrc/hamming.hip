@@ -35,6 +35,8 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int KC = 8;  // 32-bit words per row staged per pipeline step (256 bits)
 
@@ -92,8 +94,47 @@ __device__ __forceinline__ v4i expand_bits_neg(uint32_t w, int g) {
     return r;
 }
 
-// Square BM x BM output tile per workgroup, WM x WN per wavefront.
-template <int BM, int WM, int WN>
+// The MFMA helpers take the accumulator type of either kernel flavour (the
+// other flavour's body is compiled but never reached).
+template <typename Acc>
+__device__ __forceinline__ Acc mfma_i8(v4i a, v4i b, Acc c) {
+    if constexpr (std::is_same<Acc, v16i>::value) return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+    else return c;
+}
+
+// fp4 (e2m1) operands: 0 -> 0x0, +1.0 -> 0x2, -1.0 -> 0xA.  A 32-bit word of
+// one row becomes the 32 nibbles of a lane's 64-deep operand half: nibble q
+// of dword t is bit 4q + t.  Every product is 0 or +-1 and every partial sum
+// an integer of magnitude <= L <= 65,535, so the fp32 accumulation is exact.
+__device__ __forceinline__ v8i expand_fp4(uint32_t w) {
+    v8i r;
+    r[0] = (int)((w << 1) & 0x22222222u);
+    r[1] = (int)(w & 0x22222222u);
+    r[2] = (int)((w >> 1) & 0x22222222u);
+    r[3] = (int)((w >> 2) & 0x22222222u);
+    r[4] = r[5] = r[6] = r[7] = 0;
+    return r;
+}
+
+__device__ __forceinline__ v8i expand_fp4_neg(uint32_t w) {
+    v8i r = expand_fp4(w);
+#pragma unroll
+    for (int t = 0; t < 4; t++) r[t] |= r[t] << 2;  // 0x2 -> 0xA
+    return r;
+}
+
+// v_mfma_scale_f32_32x32x64_f8f6f4 with both operands fp4 (format 4) and
+// unit scales (E8M0 127 = 2^0).
+template <typename Acc>
+__device__ __forceinline__ Acc mfma_fp4(v8i a, v8i b, Acc c) {
+    if constexpr (std::is_same<Acc, v16f>::value)
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 4, 4, 0, 127, 0, 127);
+    else return c;
+}
+
+// Square BM x BM output tile per workgroup, WM x WN per wavefront; FP4
+// selects the fp4 MFMA (64 bits of k per instruction) over the int8 one (32).
+template <int BM, int WM, int WN, bool FP4>
 __global__ void __launch_bounds__((BM / WM) * (BM / WN) * 64)
 mfma_tile(TileArgs a) {
     constexpr int WAVES_N = BM / WN;
@@ -178,7 +219,8 @@ mfma_tile(TileArgs a) {
         }
     };
 
-    v16i acc[TM][TN];
+    using Acc = typename std::conditional<FP4, v16f, v16i>::type;
+    Acc acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -193,7 +235,50 @@ mfma_tile(TileArgs a) {
     // acc accumulates O O^T - P P^T: O slabs with B = +O, P slabs with
     // B = -P (the P slabs are the short ones), negated in the epilogue.
     // Words of group gi+1 are read from LDS while group gi is multiplied.
-    auto compute_slab = [&](int buf, auto neg) {
+    // fp4: each lane half g reads words 2g, 2g+1 of a 4-word group (one
+    // 8-byte read), one word per 64-deep MFMA step.
+    auto compute_slab_fp4 = [&](int buf, auto neg) {
+        uint2 wa[TM], wb[TN];
+#pragma unroll
+        for (int i = 0; i < TM; i++) wa[i] = reinterpret_cast<const uint2*>(&sA[buf][0][wm0 + i * 32 + l32])[g];
+#pragma unroll
+        for (int j = 0; j < TN; j++) wb[j] = reinterpret_cast<const uint2*>(&sB[buf][0][wn0 + j * 32 + l32])[g];
+#pragma unroll
+        for (int gi = 0; gi < KG; gi++) {
+            uint2 na[TM], nb[TN];
+            if (gi + 1 < KG) {
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+                    na[i] = reinterpret_cast<const uint2*>(&sA[buf][gi + 1][wm0 + i * 32 + l32])[g];
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    nb[j] = reinterpret_cast<const uint2*>(&sB[buf][gi + 1][wn0 + j * 32 + l32])[g];
+            }
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                v8i av[TM], bv[TN];
+#pragma unroll
+                for (int i = 0; i < TM; i++) av[i] = expand_fp4(kk ? wa[i].y : wa[i].x);
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    const uint32_t x = kk ? wb[j].y : wb[j].x;
+                    bv[j] = decltype(neg)::value ? expand_fp4_neg(x) : expand_fp4(x);
+                }
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        acc[i][j] = mfma_fp4(av[i], bv[j], acc[i][j]);
+            }
+            if (gi + 1 < KG) {
+#pragma unroll
+                for (int i = 0; i < TM; i++) wa[i] = na[i];
+#pragma unroll
+                for (int j = 0; j < TN; j++) wb[j] = nb[j];
+            }
+        }
+    };
+    auto compute_slab_i8 = [&](int buf, auto neg) {
         uint4 wa[TM], wb[TN];
 #pragma unroll
         for (int i = 0; i < TM; i++) wa[i] = sA[buf][0][wm0 + i * 32 + l32];
@@ -222,7 +307,7 @@ mfma_tile(TileArgs a) {
                 for (int i = 0; i < TM; i++)
 #pragma unroll
                     for (int j = 0; j < TN; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[i], bv[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = mfma_i8(av[i], bv[j], acc[i][j]);
             }
             if (gi + 1 < KG) {
 #pragma unroll
@@ -231,6 +316,10 @@ mfma_tile(TileArgs a) {
                 for (int j = 0; j < TN; j++) wb[j] = nb[j];
             }
         }
+    };
+    auto compute_slab = [&](int buf, auto neg) {
+        if constexpr (FP4) compute_slab_fp4(buf, neg);
+        else compute_slab_i8(buf, neg);
     };
 
     int c = 0;
@@ -260,7 +349,7 @@ mfma_tile(TileArgs a) {
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int r = rt0 + wm0 + i * 32 + (e >> 2) * 8 + g * 4 + (e & 3);
-                if (r < rows) out[(size_t)r * w + c] = (uint16_t)(c > r ? -acc[i][j][e] : 0);
+                if (r < rows) out[(size_t)r * w + c] = (uint16_t)(c > r ? -(int)acc[i][j][e] : 0);
             }
         }
 }
@@ -294,8 +383,9 @@ sub_rare_csr(uint16_t* out, int w, int r0, int rows, const uint32_t* start, cons
 // walks the flattened partner list with each lane taking every 64th partner
 // of its wavefront's span, so consecutive lanes read consecutive members and
 // RUN loads are in flight per lane.
-constexpr int RCW = 32768;  // u16 cells per chunk (64 KiB of LDS)
-constexpr int RNT = 512;    // threads per workgroup = ranges per batch
+constexpr int RCW = 65536;  // u16 cells per chunk (128 KiB of LDS: one workgroup per CU)
+constexpr int RNT = 1024;   // threads per workgroup = ranges per batch
+constexpr int RQ = (RCW / 8 + 1 + RNT - 1) / RNT;  // 16-byte words staged per thread
 constexpr int RUN = 8;      // partners per lane per pass
 __global__ void __launch_bounds__(RNT)
 sub_rare_rows(uint16_t* __restrict__ out, int n, int r0, int S, int chunks, const uint32_t* __restrict__ rstart,
@@ -318,7 +408,19 @@ sub_rare_rows(uint16_t* __restrict__ out, int n, int r0, int S, int chunks, cons
     const size_t a0 = g0 & ~(size_t)7;              // first 16-byte word touched
     const int nq = (int)((g1 - a0 + 7) >> 3);       // 16-byte words staged
     const uint4* __restrict__ src = reinterpret_cast<const uint4*>(out + a0);
-    for (int k = tid; k < nq; k += RNT) cells4[k] = src[k];
+    {  // every staging load in flight before the first LDS write
+        uint4 v[RQ];
+#pragma unroll
+        for (int u = 0; u < RQ; u++) {
+            const int k = tid + u * RNT;
+            if (k < nq) v[u] = src[k];
+        }
+#pragma unroll
+        for (int u = 0; u < RQ; u++) {
+            const int k = tid + u * RNT;
+            if (k < nq) cells4[k] = v[u];
+        }
+    }
 
     for (uint32_t eb = e0; eb < e1; eb += RNT) {
         const int nb = (int)min((uint32_t)RNT, e1 - eb);
@@ -432,11 +534,11 @@ int pick_tile(int n) { return n >= 8192 ? 256 : 128; }
 
 // Stripes r0, r0+S, ... up to row n in one launch when S is a multiple of the
 // tile (else one launch per stripe), output in the stripe layout at out.
-hipError_t launch_tiles(int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int n, int r0, int S,
-                        int ns, uint16_t* out, hipStream_t st) {
+hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int n, int r0,
+                        int S, int ns, uint16_t* out, hipStream_t st) {
     if (ns > 1 && S % tile) {
         for (int s = 0; s < ns; s++) {
-            const hipError_t e = launch_tiles(tile, O, kw1, P, kw2, n, r0 + s * S, S, 1,
+            const hipError_t e = launch_tiles(kernel, tile, O, kw1, P, kw2, n, r0 + s * S, S, 1,
                                               out + stripe_off((size_t)(n - r0), (size_t)S, (size_t)s), st);
             if (e != hipSuccess) return e;
         }
@@ -451,11 +553,22 @@ hipError_t launch_tiles(int tile, const uint32_t* O, int kw1, const uint32_t* P,
     a.T = (ns > 1 ? tiles_before(ns - 1, a.q, a.nbc) : 0) + tm_last * (a.nbc - (long long)(ns - 1) * a.q);
     a.chunk = (a.T + 7) / 8;
     const dim3 grid((unsigned)(a.chunk * 8));
-    if (tile == 256)
-        hipLaunchKernelGGL((mfma_tile<256, 128, 128>), grid, dim3(256), 0, st, a);
+    const bool fp4 = kernel == CGH_KERNEL_MFMA_FP4;
+    if (tile == 256 && fp4)
+        hipLaunchKernelGGL((mfma_tile<256, 128, 128, true>), grid, dim3(256), 0, st, a);
+    else if (tile == 256)
+        hipLaunchKernelGGL((mfma_tile<256, 128, 128, false>), grid, dim3(256), 0, st, a);
+    else if (fp4)
+        hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL((mfma_tile<128, 64, 64>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, st, a);
     return hipGetLastError();
+}
+
+// CGH_KERNEL_AUTO resolves to the fastest measured product kernel.
+int resolve_kernel(int kernel) { return kernel == CGH_KERNEL_AUTO ? CGH_KERNEL_MFMA_FP4 : kernel; }
+bool valid_kernel(int kernel) {
+    return kernel == CGH_KERNEL_AUTO || kernel == CGH_KERNEL_MFMA_I8 || kernel == CGH_KERNEL_MFMA_FP4;
 }
 
 }  // namespace
@@ -527,7 +640,8 @@ extern "C" int tch_run(int device, int n, size_t n_pad, const uint8_t* O, size_t
         if (r0 < n) {
             const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
             const size_t w = (size_t)(n - r0);
-            HK(launch_tiles(tile, dO, (int)(K1 / 32), dP, (int)(K2 / 32), n, r0, S, 1, dU[b], st[b]));
+            HK(launch_tiles(resolve_kernel(CGH_KERNEL_AUTO), tile, dO, (int)(K1 / 32), dP, (int)(K2 / 32), n, r0, S, 1,
+                            dU[b], st[b]));
             if (rare_gpu && n_rare) {
                 hipLaunchKernelGGL(sub_rare_csr, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st[b], dU[b],
                                    (int)w, r0, rows, dStart, dJ);
@@ -663,7 +777,7 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
         set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
         return 1;
     }
-    if (kernel != 0) {
+    if (!valid_kernel(kernel)) {
         set_err(err, errlen, "unknown kernel %d", kernel);
         return 1;
     }
@@ -673,7 +787,8 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
     }
     HK(hipSetDevice(ds->device));
     if (phases & CGH_PHASE_PRODUCT)
-        HK(launch_tiles(tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, 0, S, (n + S - 1) / S, d_out, st));
+        HK(launch_tiles(resolve_kernel(kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, 0, S,
+                         (n + S - 1) / S, d_out, st));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         const int chunks = (n - 1 + RCW - 1) / RCW;
         hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)n * chunks)), dim3(RNT), 0, st, d_out, n, 0, S, chunks,
@@ -721,7 +836,8 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
         if (r0 < n) {
             const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
             const size_t w = (size_t)(n - r0);
-            HK(launch_tiles(tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, r0, S, 1, dU[b], st[b]));
+            HK(launch_tiles(resolve_kernel(CGH_KERNEL_AUTO), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, r0,
+                            S, 1, dU[b], st[b]));
             if (ds->n_rare && n - r0 > 1) {
                 const int chunks = (n - r0 - 1 + RCW - 1) / RCW;
                 hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)rows * chunks)), dim3(RNT), 0, st[b], dU[b],

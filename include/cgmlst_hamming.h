@@ -141,10 +141,17 @@ size_t cgh_upper_elems(int n, int S);
 /* Offset (u16 elements) of stripe s in that layout. */
 size_t cgh_stripe_offset(int n, int S, int s);
 
+/* Binary-product kernels.  Both are exact integer arithmetic: int8 MFMA with
+ * int32 accumulation, or fp4 (e2m1, values 0 and +-1) MFMA with fp32
+ * accumulation of integers whose magnitude never exceeds L <= 65,535. */
+#define CGH_KERNEL_AUTO 0     /* the faster one on MI355X (fp4) */
+#define CGH_KERNEL_MFMA_I8 1  /* v_mfma_i32_32x32x32_i8 */
+#define CGH_KERNEL_MFMA_FP4 2 /* v_mfma_scale_f32_32x32x64_f8f6f4, fp4 operands */
+
 /* Exact H of the whole upper triangle into device memory d_out (at least
  * cgh_upper_elems(n, S) u16), stripe layout, cells with j <= i set to 0.
  * Enqueued on `stream` (a hipStream_t, NULL = default stream); returns after
- * enqueueing.  kernel selects the binary product: 0 = int8 MFMA. */
+ * enqueueing.  kernel: one of CGH_KERNEL_*. */
 int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel,
                       char* err, int errlen);
 

@@ -17,14 +17,17 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 
-def gpu_upper(cgh, torch, X, tau, S, k_align=256):
+KERNELS = [1, 2]  # CGH_KERNEL_MFMA_I8, CGH_KERNEL_MFMA_FP4
+
+
+def gpu_upper(cgh, torch, X, tau, S, k_align=256, kernel=0):
     """Encode X at tau, upload, run cgh_dataset_upper, return the square H."""
     enc = cgh.Encoded(X, tau, k_align)
     ds = cgh.Dataset(enc)
     n = X.shape[0]
     out = torch.full((cgh.upper_elems(n, S),), 0x5a5a, dtype=torch.int16, device="cuda")
     st = torch.cuda.current_stream()
-    ds.upper(S, out.data_ptr(), st.cuda_stream)
+    ds.upper(S, out.data_ptr(), st.cuda_stream, kernel)
     torch.cuda.synchronize()
     flat = out.cpu().numpy().view(np.uint16)
     ds.close()
@@ -51,15 +54,16 @@ def test_check_matrix_cli(cgh, gpu, case):
             assert np.array_equal(got, want), (case, args)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("tau", [1, 2, 8, 64, 4096])
 @pytest.mark.parametrize("S", [64, 512, 2048])
-def test_dataset_small_zipf(cgh, gpu, workloads, tau, S):
+def test_dataset_small_zipf(cgh, gpu, workloads, tau, S, kernel):
     """2,000 x 1,142 long-tail profiles, every cell, across tau and stripes."""
     calls = workloads.small_zipf(2000, 1142, seed=1)
     X = cgh.dense_ranks(calls)
     H_ref = oracle.brute_full(X)
-    print(f"dataset small_zipf tau={tau} S={S}", flush=True)
-    H, flat, enc = gpu_upper(cgh, gpu, X, tau, S)
+    print(f"dataset small_zipf tau={tau} S={S} kernel={kernel}", flush=True)
+    H, flat, enc = gpu_upper(cgh, gpu, X, tau, S, kernel=kernel)
     assert np.array_equal(H, H_ref)
 
 
@@ -76,14 +80,15 @@ def test_dataset_lower_cells_zero(cgh, gpu, workloads):
         assert not np.tril(blk[:, :rows]).any()
 
 
-def test_dataset_tile256(cgh, gpu):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_dataset_tile256(cgh, gpu, kernel):
     """n >= 8192 takes the 256 x 256 tile: random ids with missing data."""
     rng = np.random.default_rng(5)
     n, L = 9000, 200
     X = rng.integers(0, 40, size=(n, L)).astype(np.uint16)
     X[rng.random((n, L)) < 0.1] = 0
-    print("dataset tile256", flush=True)
-    H, _, _ = gpu_upper(cgh, gpu, X, 32, 4096)
+    print(f"dataset tile256 kernel={kernel}", flush=True)
+    H, _, _ = gpu_upper(cgh, gpu, X, 32, 4096, kernel=kernel)
     assert np.array_equal(H, oracle.brute_full(X))
 
 
