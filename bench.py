@@ -164,6 +164,9 @@ def main():
         if (t_rec["n"], t_rec["L"], t_rec["tau"], t_rec["stripe"]) == (N, L, tau, S):
             traffic = t_rec["kernels"]["mfma_tile"]["traffic_bytes"]
 
+    measured = cgh.measure_peaks(0)
+    log("measured peaks: " + ", ".join(f"{k} {v:.0f}" for k, v in measured.items()))
+
     flat = out.cpu().numpy().view(np.uint16)
     cpu = None if args.no_cpu_baseline else cpu_baseline(X, flat, cgh)
     if cpu is not None:
@@ -190,6 +193,8 @@ def main():
                      "kernel": "mfma_tile (binary product + u16 epilogue, one launch per step)",
                      "work": "2*(D+L) ops per pair (multiply-add per bit pair), pairs = n(n-1)/2"},
         "phases_ms": {"product": prod_ms, "rare": rare_ms},
+        "measured_peaks": measured,
+        "frac_of_measured_peak": achieved / (measured["fp4_mfma_tflops"] if kern == 2 else measured["int8_mfma_tops"]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

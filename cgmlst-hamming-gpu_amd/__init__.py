@@ -28,7 +28,7 @@ EXPORTS = (
     "cgh_dense_ranks", "cgh_encode", "cgh_rare_csr", "cgh_encoding_free",
     "cgh_allele_counts", "cgh_choose_tau",
     "cgh_dataset_upload", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
-    "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream", "cgh_dataset_tile",
+    "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream", "cgh_measure_peaks", "cgh_dataset_tile",
 )
 
 PHASE_PRODUCT, PHASE_RARE = 1, 2
@@ -82,6 +82,7 @@ def load():
         "cgh_dataset_upper": (I, [P, I, P, P, I, C.c_char_p, I]),
         "cgh_dataset_phase": (I, [P, I, P, P, I, I, C.c_char_p, I]),
         "cgh_dataset_stream": (I, [P, I, TCH_CB, P, C.POINTER(C.c_double), C.c_char_p, I]),
+        "cgh_measure_peaks": (I, [I] + [C.POINTER(C.c_double)] * 4 + [C.c_char_p, I]),
         "cgh_dataset_tile": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
@@ -111,6 +112,16 @@ def device_count() -> int:
 
 def free_vram(device: int = 0) -> int:
     return load().tch_free_vram(device)
+
+
+def measure_peaks(device: int = 0) -> dict:
+    """Register-only peaks of int8 MFMA, fp4 MFMA, VALU AND+popcount and a
+    streaming copy (cgh_measure_peaks)."""
+    v = [C.c_double() for _ in range(4)]
+    err = _err()
+    _check(load().cgh_measure_peaks(device, *[C.byref(x) for x in v], err, 512), err)
+    return {"int8_mfma_tops": v[0].value, "fp4_mfma_tflops": v[1].value, "valu_popcount_tops": v[2].value,
+            "copy_gbs": v[3].value}
 
 
 def dense_ranks(calls: np.ndarray) -> np.ndarray:

@@ -141,14 +141,16 @@ mfma_tile(TileArgs a) {
     constexpr int NT = (BM / WM) * WAVES_N * 64;
     constexpr int TM = WM / 32, TN = WN / 32;
     constexpr int KG = KC / 4;                  // 4-word groups per slab
-    constexpr int LOADS = 2 * BM * KG;          // 16-byte loads per slab (A and B)
-    constexpr int LPT = LOADS / NT;
-    static_assert(LOADS % NT == 0, "slab loads must divide evenly");
+    constexpr int NST = 4;                      // LDS ring stages (slabs in flight)
+    constexpr int WLOADS = 2 * KG * (BM / 64);  // wavefront-wide 16-byte loads per slab
+    constexpr int LPW = WLOADS / (NT / 64);     // of which each wavefront issues
+    static_assert(WLOADS % (NT / 64) == 0, "slab loads must divide evenly");
 
-    // [buffer][group][row] of 4 consecutive words: a wavefront's 16-byte
-    // reads of 32 consecutive rows are contiguous, hence conflict-free.
-    __shared__ uint4 sA[2][KG][BM];
-    __shared__ uint4 sB[2][KG][BM];
+    // ring[stage][side A/B][group][row] of 4 consecutive words: one
+    // wavefront-wide direct-to-LDS load fills 64 consecutive rows of a group
+    // (1 KiB, contiguous), and a wavefront's 16-byte reads of 32 consecutive
+    // rows are contiguous, hence conflict-free.
+    __shared__ uint4 ring[NST][2][KG][BM];
 
     const int tid = threadIdx.x;
     // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
@@ -189,35 +191,41 @@ mfma_tile(TileArgs a) {
     const int nc1 = a.kw1 / KC, nc = nc1 + a.kw2 / KC;
     const int growA = sr0 + rt0, growB = sr0 + ct0;
 
-    uint4 stage[LPT];
-    auto load_slab = [&](int c) {
+    // Slab c (KC words of every A and B row of the tile) -> ring stage st,
+    // straight from global memory into LDS (global_load_lds_dwordx4: lane l
+    // of the wavefront lands at the wavefront's LDS base + 16 l).  Rows past
+    // n read row n-1: their products only reach output cells that are never
+    // stored.
+    auto issue_slab = [&](int c, int st) {
         const uint32_t* base;
         int kw, k0;
         if (c < nc1) { base = a.O; kw = a.kw1; k0 = c * KC; }
         else { base = a.P; kw = a.kw2; k0 = (c - nc1) * KC; }
 #pragma unroll
-        for (int u = 0; u < LPT; u++) {
-            const int q = tid + u * NT;
-            const int side = q / (BM * KG);
-            const int qq = q % (BM * KG);
-            const int grp = qq / BM, row = qq % BM;
-            const int grow = (side ? growB : growA) + row;
-            if (grow < a.n)
-                stage[u] = *(const uint4*)(base + (size_t)grow * kw + k0 + grp * 4);
-            else
-                stage[u] = make_uint4(0, 0, 0, 0);
+        for (int u = 0; u < LPW; u++) {
+            const int v = wave * LPW + u;
+            const int side = v / (KG * (BM / 64));
+            const int rem = v % (KG * (BM / 64));
+            const int grp = rem / (BM / 64), rb = rem % (BM / 64);
+            const int grow = min((side ? growB : growA) + rb * 64 + lane, a.n - 1);
+            const uint32_t* src = base + (size_t)grow * kw + k0 + grp * 4;
+            // LDS base of this wavefront's 64 rows goes to M0.  Written as an
+            // instruction rather than the builtin so that the compiler does
+            // not wait for every LDS-DMA transfer before each LDS read (it
+            // cannot tell the ring stages apart); the waits are explicit
+            // below.  The load has no register result.
+            const uint32_t lds = __builtin_amdgcn_readfirstlane(
+                (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)&ring[st][side][grp][rb * 64]);
+            asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(src), "s"(lds) : "memory", "m0");
         }
     };
-    auto store_slab = [&](int buf) {
-#pragma unroll
-        for (int u = 0; u < LPT; u++) {
-            const int q = tid + u * NT;
-            const int side = q / (BM * KG);
-            const int qq = q % (BM * KG);
-            const int grp = qq / BM, row = qq % BM;
-            (side ? sB : sA)[buf][grp][row] = stage[u];
-        }
-    };
+    // Every step issues exactly one slab (past the last slab it reloads slab
+    // nc-1 into the stage nobody reads again), so "slab c has landed" is
+    // always "at most NST-2 slabs of this wavefront's loads pending": one
+    // constant wait, no branch in the loop body.
+    constexpr int AHEAD = (NST - 2) * LPW;
+    static_assert(AHEAD < 64, "vmcnt field is 6 bits");
+    constexpr int WAIT_AHEAD = (AHEAD & 15) | ((AHEAD >> 4) << 14) | (7 << 4) | (15 << 8);
 
     using Acc = typename std::conditional<FP4, v16f, v16i>::type;
     Acc acc[TM][TN];
@@ -228,9 +236,8 @@ mfma_tile(TileArgs a) {
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[i][j][e] = 0;
 
-    load_slab(0);
-    store_slab(0);
-    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NST - 1; p++) issue_slab(min(p, nc - 1), p);
 
     // acc accumulates O O^T - P P^T: O slabs with B = +O, P slabs with
     // B = -P (the P slabs are the short ones), negated in the epilogue.
@@ -240,19 +247,19 @@ mfma_tile(TileArgs a) {
     auto compute_slab_fp4 = [&](int buf, auto neg) {
         uint2 wa[TM], wb[TN];
 #pragma unroll
-        for (int i = 0; i < TM; i++) wa[i] = reinterpret_cast<const uint2*>(&sA[buf][0][wm0 + i * 32 + l32])[g];
+        for (int i = 0; i < TM; i++) wa[i] = reinterpret_cast<const uint2*>(&ring[buf][0][0][wm0 + i * 32 + l32])[g];
 #pragma unroll
-        for (int j = 0; j < TN; j++) wb[j] = reinterpret_cast<const uint2*>(&sB[buf][0][wn0 + j * 32 + l32])[g];
+        for (int j = 0; j < TN; j++) wb[j] = reinterpret_cast<const uint2*>(&ring[buf][1][0][wn0 + j * 32 + l32])[g];
 #pragma unroll
         for (int gi = 0; gi < KG; gi++) {
             uint2 na[TM], nb[TN];
             if (gi + 1 < KG) {
 #pragma unroll
                 for (int i = 0; i < TM; i++)
-                    na[i] = reinterpret_cast<const uint2*>(&sA[buf][gi + 1][wm0 + i * 32 + l32])[g];
+                    na[i] = reinterpret_cast<const uint2*>(&ring[buf][0][gi + 1][wm0 + i * 32 + l32])[g];
 #pragma unroll
                 for (int j = 0; j < TN; j++)
-                    nb[j] = reinterpret_cast<const uint2*>(&sB[buf][gi + 1][wn0 + j * 32 + l32])[g];
+                    nb[j] = reinterpret_cast<const uint2*>(&ring[buf][1][gi + 1][wn0 + j * 32 + l32])[g];
             }
 #pragma unroll
             for (int kk = 0; kk < 2; kk++) {
@@ -281,17 +288,17 @@ mfma_tile(TileArgs a) {
     auto compute_slab_i8 = [&](int buf, auto neg) {
         uint4 wa[TM], wb[TN];
 #pragma unroll
-        for (int i = 0; i < TM; i++) wa[i] = sA[buf][0][wm0 + i * 32 + l32];
+        for (int i = 0; i < TM; i++) wa[i] = ring[buf][0][0][wm0 + i * 32 + l32];
 #pragma unroll
-        for (int j = 0; j < TN; j++) wb[j] = sB[buf][0][wn0 + j * 32 + l32];
+        for (int j = 0; j < TN; j++) wb[j] = ring[buf][1][0][wn0 + j * 32 + l32];
 #pragma unroll
         for (int gi = 0; gi < KG; gi++) {
             uint4 na[TM], nb[TN];
             if (gi + 1 < KG) {
 #pragma unroll
-                for (int i = 0; i < TM; i++) na[i] = sA[buf][gi + 1][wm0 + i * 32 + l32];
+                for (int i = 0; i < TM; i++) na[i] = ring[buf][0][gi + 1][wm0 + i * 32 + l32];
 #pragma unroll
-                for (int j = 0; j < TN; j++) nb[j] = sB[buf][gi + 1][wn0 + j * 32 + l32];
+                for (int j = 0; j < TN; j++) nb[j] = ring[buf][1][gi + 1][wn0 + j * 32 + l32];
             }
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) {
@@ -322,21 +329,22 @@ mfma_tile(TileArgs a) {
         else compute_slab_i8(buf, neg);
     };
 
+    // Slab c: wait for its loads, barrier (it is visible to every wavefront
+    // and stage (c-1) % NST is free), refill that stage with slab c+NST-1,
+    // multiply slab c.
+    auto step = [&](int c, auto neg) {
+        __builtin_amdgcn_s_waitcnt(WAIT_AHEAD);
+        // every LDS read of the previous slab has returned, then the
+        // workgroup barrier (no memory fence: the DMA waits are explicit)
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" : : : "memory");
+        issue_slab(min(c + NST - 1, nc - 1), (c + NST - 1) % NST);
+        compute_slab(c % NST, neg);
+    };
     int c = 0;
-    for (; c < nc1; c++) {
-        const int buf = c & 1;
-        load_slab(c + 1);  // nc1 < nc: the P slabs always follow
-        compute_slab(buf, std::false_type{});
-        store_slab(buf ^ 1);
-        __syncthreads();
-    }
-    for (; c < nc; c++) {
-        const int buf = c & 1;
-        if (c + 1 < nc) load_slab(c + 1);
-        compute_slab(buf, std::true_type{});
-        if (c + 1 < nc) store_slab(buf ^ 1);
-        __syncthreads();
-    }
+    for (; c < nc1; c++) step(c, std::false_type{});
+    for (; c < nc; c++) step(c, std::true_type{});
+    // the trailing reloads must land before the workgroup's LDS is released
+    __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
 
     // Epilogue: lane holds column l32 and rows 8*(e/4) + 4*g + (e%4) of
     // O O^T - P P^T; the distance is its negation.

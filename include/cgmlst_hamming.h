@@ -171,6 +171,13 @@ int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int
  * no counterpart in the reference beyond tch_run's shape. */
 int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, char* err, int errlen);
 
+/* Measured peaks of the units the kernels use, from register-only loops on
+ * `device`: int8 MFMA (TOP/s), fp4 MFMA (TFLOP/s), VALU AND + popcount
+ * (TOP/s, 2 ops per bit pair) and a streaming copy (GB/s, read + write).
+ * Any output pointer may be NULL.  Same role as tensorcore/peak.cu:8-54. */
+int cgh_measure_peaks(int device, double* i8_tops, double* fp4_tflops, double* popc_tops, double* hbm_gbs,
+                      char* err, int errlen);
+
 /* Tile shape (rows = cols) used by cgh_dataset_upper for this dataset. */
 int cgh_dataset_tile(const cgh_dataset* ds, int kernel);
 
