@@ -28,7 +28,9 @@ EXPORTS = (
     "cgh_dense_ranks", "cgh_encode", "cgh_rare_csr", "cgh_encoding_free",
     "cgh_allele_counts", "cgh_choose_tau",
     "cgh_dataset_upload", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
-    "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream", "cgh_measure_peaks", "cgh_dataset_tile",
+    "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream",
+    "cgh_batch_create", "cgh_batch_out_elems", "cgh_batch_out_offset", "cgh_batch_tile", "cgh_batch_run",
+    "cgh_batch_free", "cgh_measure_peaks", "cgh_dataset_tile",
 )
 
 PHASE_PRODUCT, PHASE_RARE = 1, 2
@@ -83,6 +85,12 @@ def load():
         "cgh_dataset_phase": (I, [P, I, P, P, I, I, C.c_char_p, I]),
         "cgh_dataset_stream": (I, [P, I, TCH_CB, P, C.POINTER(C.c_double), C.c_char_p, I]),
         "cgh_measure_peaks": (I, [I] + [C.POINTER(C.c_double)] * 4 + [C.c_char_p, I]),
+        "cgh_batch_create": (I, [I, C.POINTER(P), I, I, I, C.POINTER(P), C.c_char_p, I]),
+        "cgh_batch_out_elems": (S, [P]),
+        "cgh_batch_out_offset": (S, [P, I]),
+        "cgh_batch_tile": (I, [P, I]),
+        "cgh_batch_run": (I, [P, P, P, I, I, C.c_char_p, I]),
+        "cgh_batch_free": (None, [P]),
         "cgh_dataset_tile": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
@@ -273,6 +281,39 @@ class Dataset:
         err = _err()
         _check(load().cgh_dataset_phase(self.handle, S, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), kernel, phases,
                                         err, 512), err)
+
+
+class Batch:
+    """Many uploaded datasets in one persistent launch (cgh_batch_*)."""
+
+    def __init__(self, datasets, S: int, tile: int = 0, device: int = 0):
+        self.datasets = list(datasets)  # keep them alive
+        arr = (C.c_void_p * len(self.datasets))(*[d.handle for d in self.datasets])
+        self.handle = C.c_void_p()
+        err = _err()
+        _check(load().cgh_batch_create(device, arr, len(self.datasets), S, tile, C.byref(self.handle), err, 512), err)
+        self.S = S
+
+    def close(self):
+        if _lib is not None and self.handle:
+            _lib.cgh_batch_free(self.handle)
+            self.handle = C.c_void_p()
+
+    __del__ = close
+
+    def out_elems(self) -> int:
+        return load().cgh_batch_out_elems(self.handle)
+
+    def offset(self, d: int) -> int:
+        return load().cgh_batch_out_offset(self.handle, d)
+
+    def tile(self, d: int) -> int:
+        return load().cgh_batch_tile(self.handle, d)
+
+    def run(self, out_ptr: int, stream_ptr: int = 0, kernel: int = 0, graph: bool = False):
+        err = _err()
+        _check(load().cgh_batch_run(self.handle, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), kernel, int(graph), err,
+                                    512), err)
 
 
 def upper_to_square(flat: np.ndarray, n: int, S: int) -> np.ndarray:

@@ -28,6 +28,7 @@
 #include <string.h>
 #include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "cgmlst_hamming.h"
 
@@ -47,19 +48,23 @@ constexpr int KC = 8;  // 32-bit words per row staged per pipeline step (256 bit
 // J = 0..ceil(w/BM)-1, numbered stripe by stripe, row-major.  With ns > 1, S
 // is a multiple of BM, so a full stripe has q = S/BM row tiles and nbc - s*q
 // column tiles, and the tile count before stripe s has a closed form.
-struct TileArgs {
+struct TileJob {
     const uint32_t* O;  // n_pad x kw1 words (frequent-allele one-hot bits)
     const uint32_t* P;  // n_pad x kw2 words (presence bits)
     int kw1, kw2;       // words per row, multiples of KC
     int n;              // valid rows
-    int r0;             // first row of the launch
+    int r0;             // first row of the layout
     int S;              // stripe height
+    uint16_t* out;      // stripe layout base
+};
+
+struct TileArgs {
+    TileJob job;
     int ns;             // stripes in the launch
     int q;              // row tiles of a full stripe (ns > 1)
     int nbc;            // column tiles of the first stripe
     long long T;        // tiles in the launch
     long long chunk;    // tiles per XCD: consecutive tiles share a row panel in one L2
-    uint16_t* out;      // stripe layout base
 };
 
 // Offset (u16 elements) of stripe s in the concatenated upper-triangle layout.
@@ -134,42 +139,27 @@ __device__ __forceinline__ Acc mfma_fp4(v8i a, v8i b, Acc c) {
 
 // Square BM x BM output tile per workgroup, WM x WN per wavefront; FP4
 // selects the fp4 MFMA (64 bits of k per instruction) over the int8 one (32).
+constexpr int KG = KC / 4;   // 4-word groups per slab
+constexpr int NST = 4;       // LDS ring stages (slabs in flight)
+
+// ring[stage][side A/B][group][row] of 4 consecutive words: one
+// wavefront-wide direct-to-LDS load fills 64 consecutive rows of a group
+// (1 KiB, contiguous), and a wavefront's 16-byte reads of 32 consecutive
+// rows are contiguous, hence conflict-free.
+template <int BM>
+using Ring = uint4[NST][2][KG][BM];
+
+// One BM x BM tile (I, J) of stripe s of job a, WM x WN per wavefront.
 template <int BM, int WM, int WN, bool FP4>
-__global__ void __launch_bounds__((BM / WM) * (BM / WN) * 64)
-mfma_tile(TileArgs a) {
+__device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J, Ring<BM>& ring) {
     constexpr int WAVES_N = BM / WN;
     constexpr int NT = (BM / WM) * WAVES_N * 64;
     constexpr int TM = WM / 32, TN = WN / 32;
-    constexpr int KG = KC / 4;                  // 4-word groups per slab
-    constexpr int NST = 4;                      // LDS ring stages (slabs in flight)
     constexpr int WLOADS = 2 * KG * (BM / 64);  // wavefront-wide 16-byte loads per slab
     constexpr int LPW = WLOADS / (NT / 64);     // of which each wavefront issues
     static_assert(WLOADS % (NT / 64) == 0, "slab loads must divide evenly");
 
-    // ring[stage][side A/B][group][row] of 4 consecutive words: one
-    // wavefront-wide direct-to-LDS load fills 64 consecutive rows of a group
-    // (1 KiB, contiguous), and a wavefront's 16-byte reads of 32 consecutive
-    // rows are contiguous, hence conflict-free.
-    __shared__ uint4 ring[NST][2][KG][BM];
-
     const int tid = threadIdx.x;
-    // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
-    // contiguous run of tiles so concurrently resident tiles share panels in
-    // that XCD's L2.
-    const long long t = (long long)(blockIdx.x & 7) * a.chunk + (blockIdx.x >> 3);
-    if (t >= a.T) return;
-    int s = 0;
-    if (a.ns > 1) {
-        int lo = 0, hi = a.ns - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (tiles_before(mid, a.q, a.nbc) <= t) lo = mid; else hi = mid - 1;
-        }
-        s = lo;
-    }
-    const long long local = t - (s ? tiles_before(s, a.q, a.nbc) : 0);
-    const int tn = a.nbc - s * a.q;
-    const int I = (int)(local / tn), J = (int)(local % tn);
     const int sr0 = a.r0 + s * a.S;              // first row of the stripe
     const int rows = min(a.S, a.n - sr0), w = a.n - sr0;
     uint16_t* __restrict__ out = a.out + stripe_off((size_t)(a.n - a.r0), (size_t)a.S, (size_t)s);
@@ -362,6 +352,53 @@ mfma_tile(TileArgs a) {
         }
 }
 
+// One launch over the closed-form tile numbering of TileArgs.  Workgroups
+// are dealt round-robin to the 8 XCDs; each XCD gets a contiguous run of
+// tiles so concurrently resident tiles share panels in that XCD's L2.
+template <int BM, int WM, int WN, bool FP4>
+__global__ void __launch_bounds__((BM / WM) * (BM / WN) * 64)
+mfma_tile(TileArgs a) {
+    __shared__ Ring<BM> ring;
+    const long long t = (long long)(blockIdx.x & 7) * a.chunk + (blockIdx.x >> 3);
+    if (t >= a.T) return;
+    int s = 0;
+    if (a.ns > 1) {
+        int lo = 0, hi = a.ns - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tiles_before(mid, a.q, a.nbc) <= t) lo = mid; else hi = mid - 1;
+        }
+        s = lo;
+    }
+    const long long local = t - (s ? tiles_before(s, a.q, a.nbc) : 0);
+    const int tn = a.nbc - s * a.q;
+    tile_body<BM, WM, WN, FP4>(a.job, s, (int)(local / tn), (int)(local % tn), ring);
+}
+
+// Persistent form for many datasets: each workgroup pulls tiles (job,
+// stripe, I, J) from a device work queue until the list is exhausted; the
+// list is ordered longest-first, so uneven datasets finish together.
+template <int BM, int WM, int WN, bool FP4>
+__global__ void __launch_bounds__((BM / WM) * (BM / WN) * 64)
+mfma_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* __restrict__ outoff,
+                const int4* __restrict__ tiles, int T, int* __restrict__ next) {
+    __shared__ Ring<BM> ring;
+    __shared__ int slot;
+    for (;;) {
+        if (threadIdx.x == 0) slot = atomicAdd(next, 1);
+        __syncthreads();
+        const int t = slot;
+        __syncthreads();  // every wavefront has read slot, and left the previous tile
+        if (t >= T) break;
+        const int4 tl = tiles[t];
+        TileJob jb = jobs[tl.x];
+        jb.out = base + outoff[tl.x];
+        tile_body<BM, WM, WN, FP4>(jb, tl.y, tl.z, tl.w, ring);
+    }
+}
+
+__global__ void reset_counter(int* c) { *c = 0; }
+
 // Subtract one from the u16 cell at element offset off of a 4-byte aligned
 // u16 array.  The cell never goes below the value it ends with, so the
 // half-word arithmetic inside the 32-bit atomic never borrows.
@@ -395,16 +432,36 @@ constexpr int RCW = 65536;  // u16 cells per chunk (128 KiB of LDS: one workgrou
 constexpr int RNT = 1024;   // threads per workgroup = ranges per batch
 constexpr int RQ = (RCW / 8 + 1 + RNT - 1) / RNT;  // 16-byte words staged per thread
 constexpr int RUN = 8;      // partners per lane per pass
-__global__ void __launch_bounds__(RNT)
-sub_rare_rows(uint16_t* __restrict__ out, int n, int r0, int S, int chunks, const uint32_t* __restrict__ rstart,
-              const uint64_t* __restrict__ rentry, const uint32_t* __restrict__ gmem) {
-    __shared__ uint4 cells4[RCW / 8 + 1];
-    __shared__ uint32_t ebeg[RNT];
-    __shared__ uint32_t epre[RNT + 1];
-    __shared__ uint32_t wsum[RNT / 64];
+struct RareJob {
+    uint16_t* out;             // stripe layout of rows r0..
+    int n, r0, S, chunks;      // chunks = column chunks per row
+    const uint32_t* rstart;    // n + 1 offsets into rentry
+    const uint64_t* rentry;    // partner ranges of gmem
+    const uint32_t* gmem;      // group members
+    long long block0;          // first workgroup of this job in a batched launch
+    int d;                     // dataset of this job in a batched launch
+};
+
+struct RareShared {
+    uint4 cells4[RCW / 8 + 1];
+    uint32_t ebeg[RNT];
+    uint32_t epre[RNT + 1];
+    uint32_t wsum[RNT / 64];
+};
+
+// Row i (>= r0), column chunk ch of job jb.
+__device__ __forceinline__ void rare_rows_body(const RareJob& jb, int i, int ch, RareShared& sh) {
+    uint16_t* __restrict__ out = jb.out;
+    const int n = jb.n, r0 = jb.r0, S = jb.S, chunks = jb.chunks;
+    const uint32_t* __restrict__ rstart = jb.rstart;
+    const uint64_t* __restrict__ rentry = jb.rentry;
+    const uint32_t* __restrict__ gmem = jb.gmem;
+    uint4* cells4 = sh.cells4;
+    uint32_t* ebeg = sh.ebeg;
+    uint32_t* epre = sh.epre;
+    uint32_t* wsum = sh.wsum;
     uint32_t* cells = reinterpret_cast<uint32_t*>(cells4);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i = r0 + (int)(blockIdx.x / chunks), ch = blockIdx.x % chunks;
     const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
     const int j0 = i + 1 + ch * RCW, j1 = min(n, j0 + RCW);
     if (e0 == e1 || j0 >= n) return;
@@ -516,6 +573,29 @@ sub_rare_rows(uint16_t* __restrict__ out, int n, int r0, int S, int chunks, cons
     }
 }
 
+// Rows r0 .. r0 + gridDim.x / chunks - 1 of one job.
+__global__ void __launch_bounds__(RNT) sub_rare_rows(RareJob jb) {
+    __shared__ RareShared sh;
+    rare_rows_body(jb, jb.r0 + (int)(blockIdx.x / jb.chunks), (int)(blockIdx.x % jb.chunks), sh);
+}
+
+// Many jobs in one launch: workgroup b belongs to the last job whose
+// block0 <= b (jobs ordered by block0).
+__global__ void __launch_bounds__(RNT) sub_rare_rows_batch(const RareJob* __restrict__ jobs, int njobs,
+                                                            uint16_t* base, const size_t* __restrict__ outoff) {
+    __shared__ RareShared sh;
+    const long long b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].block0 <= b) lo = mid; else hi = mid - 1;
+    }
+    RareJob jb = jobs[lo];
+    jb.out = base + outoff[jb.d];
+    const long long local = b - jb.block0;
+    rare_rows_body(jb, jb.r0 + (int)(local / jb.chunks), (int)(local % jb.chunks), sh);
+}
+
 size_t up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
 void set_err(char* err, int errlen, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
@@ -553,7 +633,9 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
         return hipSuccess;
     }
     TileArgs a;
-    a.O = O; a.P = P; a.kw1 = kw1; a.kw2 = kw2; a.n = n; a.r0 = r0; a.S = S; a.ns = ns; a.out = out;
+    a.job.O = O; a.job.P = P; a.job.kw1 = kw1; a.job.kw2 = kw2; a.job.n = n; a.job.r0 = r0; a.job.S = S;
+    a.job.out = out;
+    a.ns = ns;
     a.nbc = (n - r0 + tile - 1) / tile;
     a.q = S / tile;
     const int last = r0 + (ns - 1) * S;
@@ -799,8 +881,8 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
                          (n + S - 1) / S, d_out, st));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         const int chunks = (n - 1 + RCW - 1) / RCW;
-        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)n * chunks)), dim3(RNT), 0, st, d_out, n, 0, S, chunks,
-                           ds->drstart, ds->drentry, ds->dgmem);
+        const RareJob jb = {d_out, n, 0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0};
+        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)n * chunks)), dim3(RNT), 0, st, jb);
         HK(hipGetLastError());
     }
     return 0;
@@ -811,6 +893,198 @@ fail:
 extern "C" int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, char* err,
                                  int errlen) {
     return cgh_dataset_phase(ds, S, d_out, stream, kernel, CGH_PHASE_PRODUCT | CGH_PHASE_RARE, err, errlen);
+}
+
+// ---------------------------------------------------------------------------
+// Many independent datasets in one persistent launch
+// ---------------------------------------------------------------------------
+
+struct cgh_batch {
+    int device, count, S;
+    int cus = 0;                        // compute units: persistent grid size
+    size_t total;                       // u16 elements of all outputs
+    std::vector<size_t> offs;           // output offset of each dataset (16-byte aligned)
+    std::vector<int> tile;              // tile edge chosen per dataset
+    TileJob* djobs = NULL;              // per dataset (out filled in-kernel)
+    size_t* doffs = NULL;
+    int4* dtiles[2] = {NULL, NULL};     // work lists of the 256 and 128 tile kernels
+    int ntiles[2] = {0, 0};
+    RareJob* drare = NULL;
+    int nrare = 0;
+    long long rare_blocks = 0;
+    int* dcount = NULL;                 // two work-queue counters
+    hipGraphExec_t exec = NULL;         // captured launch sequence
+    uint16_t* graph_out = NULL;
+    hipStream_t graph_stream = NULL;
+    int graph_kernel = -1;
+};
+
+extern "C" void cgh_batch_free(cgh_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->djobs) (void)hipFree(b->djobs);
+    if (b->doffs) (void)hipFree(b->doffs);
+    for (int k = 0; k < 2; k++)
+        if (b->dtiles[k]) (void)hipFree(b->dtiles[k]);
+    if (b->drare) (void)hipFree(b->drare);
+    if (b->dcount) (void)hipFree(b->dcount);
+    delete b;
+}
+
+extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count, int S, int tile, cgh_batch** out,
+                                char* err, int errlen) {
+    cgh_batch* b = new cgh_batch();
+    std::vector<TileJob> jobs((size_t)count);
+    std::vector<RareJob> rjobs;
+    std::vector<std::pair<long long, int4>> lists[2];  // (cost, tile) per kernel
+    *out = NULL;
+    b->device = device;
+    b->count = count;
+    b->S = S;
+    b->total = 0;
+    if (count < 1 || S < 8 || S % 8 || (tile != 0 && tile != 128 && tile != 256)) {
+        set_err(err, errlen, "cgh_batch_create: bad arguments (count=%d S=%d tile=%d)", count, S, tile);
+        delete b;
+        return 1;
+    }
+    for (int d = 0; d < count; d++) {
+        const cgh_dataset* ds = sets[d];
+        if (ds->device != device) {
+            set_err(err, errlen, "dataset %d lives on device %d, not %d", d, ds->device, device);
+            delete b;
+            return 1;
+        }
+        // per-dataset tile: 256 when its triangle holds enough 256-tiles to
+        // fill the chip, else 128
+        const int tl = tile ? tile : (ds->n >= 4096 ? 256 : 128);
+        b->tile.push_back(tl);
+        b->offs.push_back(b->total);
+        b->total += cgh_upper_elems(ds->n, S);
+        jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL};
+        const int k = tl == 256 ? 0 : 1;
+        const long long cost = (long long)(ds->kw1 + ds->kw2);
+        for (int s = 0, r0 = 0; r0 < ds->n; s++, r0 += S) {
+            const int rows = std::min(S, ds->n - r0), w = ds->n - r0;
+            for (int I = 0; I * tl < rows; I++)
+                for (int J = 0; J * tl < w; J++) lists[k].push_back({J >= I ? cost : 0, make_int4(d, s, I, J)});
+        }
+        if (ds->n_rare && ds->n > 1) {
+            RareJob r = {NULL, ds->n, 0, S, (ds->n - 1 + RCW - 1) / RCW, ds->drstart, ds->drentry, ds->dgmem,
+                         b->rare_blocks, d};
+            rjobs.push_back(r);
+            b->rare_blocks += (long long)ds->n * r.chunks;
+        }
+    }
+    b->nrare = (int)rjobs.size();
+    HK(hipSetDevice(device));
+    HK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
+    HK(hipMalloc(&b->djobs, sizeof(TileJob) * (size_t)count));
+    HK(hipMemcpy(b->djobs, jobs.data(), sizeof(TileJob) * (size_t)count, hipMemcpyHostToDevice));
+    HK(hipMalloc(&b->doffs, sizeof(size_t) * (size_t)count));
+    HK(hipMemcpy(b->doffs, b->offs.data(), sizeof(size_t) * (size_t)count, hipMemcpyHostToDevice));
+    for (int k = 0; k < 2; k++) {
+        // longest tiles first; below-diagonal (zero-fill) tiles last
+        std::stable_sort(lists[k].begin(), lists[k].end(),
+                         [](const std::pair<long long, int4>& x, const std::pair<long long, int4>& y) {
+                             return x.first > y.first;
+                         });
+        std::vector<int4> t;
+        for (auto& e : lists[k]) t.push_back(e.second);
+        b->ntiles[k] = (int)t.size();
+        if (t.empty()) continue;
+        HK(hipMalloc(&b->dtiles[k], sizeof(int4) * t.size()));
+        HK(hipMemcpy(b->dtiles[k], t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
+    }
+    if (b->nrare) {
+        HK(hipMalloc(&b->drare, sizeof(RareJob) * rjobs.size()));
+        HK(hipMemcpy(b->drare, rjobs.data(), sizeof(RareJob) * rjobs.size(), hipMemcpyHostToDevice));
+    }
+    HK(hipMalloc(&b->dcount, 2 * sizeof(int)));
+    *out = b;
+    return 0;
+fail:
+    cgh_batch_free(b);
+    return 1;
+}
+
+extern "C" size_t cgh_batch_out_elems(const cgh_batch* b) { return b->total; }
+extern "C" size_t cgh_batch_out_offset(const cgh_batch* b, int d) { return b->offs[(size_t)d]; }
+extern "C" int cgh_batch_tile(const cgh_batch* b, int d) { return b->tile[(size_t)d]; }
+
+namespace {
+
+// The whole batch: counter resets, the persistent product kernel per tile
+// size, then one rare-pair launch over every dataset's rows.
+hipError_t batch_enqueue(cgh_batch* b, uint16_t* d_out, hipStream_t st, int kernel) {
+    const bool fp4 = resolve_kernel(kernel) == CGH_KERNEL_MFMA_FP4;
+    const unsigned cus = (unsigned)b->cus;
+    for (int k = 0; k < 2; k++) hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, b->dcount + k);
+    if (b->ntiles[0]) {
+        if (fp4)
+            hipLaunchKernelGGL((mfma_tile_queue<256, 128, 128, true>), dim3(cus), dim3(256), 0, st, b->djobs, d_out,
+                               b->doffs, b->dtiles[0], b->ntiles[0], b->dcount);
+        else
+            hipLaunchKernelGGL((mfma_tile_queue<256, 128, 128, false>), dim3(cus), dim3(256), 0, st, b->djobs, d_out,
+                               b->doffs, b->dtiles[0], b->ntiles[0], b->dcount);
+    }
+    if (b->ntiles[1]) {
+        if (fp4)
+            hipLaunchKernelGGL((mfma_tile_queue<128, 64, 64, true>), dim3(cus * 2), dim3(256), 0, st, b->djobs,
+                               d_out, b->doffs, b->dtiles[1], b->ntiles[1], b->dcount + 1);
+        else
+            hipLaunchKernelGGL((mfma_tile_queue<128, 64, 64, false>), dim3(cus * 2), dim3(256), 0, st, b->djobs,
+                               d_out, b->doffs, b->dtiles[1], b->ntiles[1], b->dcount + 1);
+    }
+    if (b->nrare)
+        hipLaunchKernelGGL(sub_rare_rows_batch, dim3((unsigned)b->rare_blocks), dim3(RNT), 0, st, b->drare, b->nrare,
+                           d_out, b->doffs);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int kernel, int use_graph, char* err,
+                             int errlen) {
+    hipStream_t st = (hipStream_t)stream;
+    hipGraph_t graph = NULL;
+    if (!valid_kernel(kernel)) {
+        set_err(err, errlen, "unknown kernel %d", kernel);
+        return 1;
+    }
+    if (((uintptr_t)d_out) & 15) {
+        set_err(err, errlen, "output buffer must be 16-byte aligned");
+        return 1;
+    }
+    HK(hipSetDevice(b->device));
+    if (!use_graph || st == NULL) {
+        HK(batch_enqueue(b, d_out, st, kernel));
+        return 0;
+    }
+    if (!b->exec || b->graph_out != d_out || b->graph_stream != st || b->graph_kernel != kernel) {
+        if (b->exec) {
+            HK(hipGraphExecDestroy(b->exec));
+            b->exec = NULL;
+        }
+        HK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        {
+            const hipError_t e = batch_enqueue(b, d_out, st, kernel);
+            const hipError_t e2 = hipStreamEndCapture(st, &graph);
+            HK(e);
+            HK(e2);
+        }
+        HK(hipGraphInstantiate(&b->exec, graph, NULL, NULL, 0));
+        HK(hipGraphDestroy(graph));
+        graph = NULL;
+        b->graph_out = d_out;
+        b->graph_stream = st;
+        b->graph_kernel = kernel;
+    }
+    HK(hipGraphLaunch(b->exec, st));
+    return 0;
+fail:
+    if (graph) (void)hipGraphDestroy(graph);
+    return 1;
 }
 
 // Stripes streamed to the host: the reference pipeline's shape (tch.cu:82-170,
@@ -848,8 +1122,8 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
                             S, 1, dU[b], st[b]));
             if (ds->n_rare && n - r0 > 1) {
                 const int chunks = (n - r0 - 1 + RCW - 1) / RCW;
-                hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)rows * chunks)), dim3(RNT), 0, st[b], dU[b],
-                                   n, r0, S, chunks, ds->drstart, ds->drentry, ds->dgmem);
+                const RareJob jb = {dU[b], n, r0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0};
+                hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)rows * chunks)), dim3(RNT), 0, st[b], jb);
                 HK(hipGetLastError());
             }
             HK(hipMemcpyAsync(hU[b], dU[b], (size_t)rows * w * 2, hipMemcpyDeviceToHost, st[b]));

@@ -56,6 +56,26 @@ def long_tail(n=63005, L=1142, seed=2, a_min=5, a_max=3000, s=1.1):
     return out
 
 
+def zipf_iid(n, L, seed, s=1.2, a_min=5, a_max=3000, miss=0.02):
+    """Long-tail alleles (log-uniform allele counts per locus, Zipf(s)
+    frequencies) with i.i.d. missing calls."""
+    rng = np.random.default_rng(seed)
+    A_k = np.rint(np.exp(rng.uniform(np.log(a_min), np.log(a_max), size=L))).astype(np.int64)
+    out = np.zeros((n, L), dtype=np.uint64)
+    for k in range(L):
+        out[:, k] = _zipf_locus(rng, n, int(A_k[k]), s, rng.random(n) < miss)
+    return out
+
+
+def batch_shapes(count=64, seed=4, n_range=(500, 20000), L_range=(800, 3000)):
+    """(n, L, seed) of the independent datasets of the batching workload:
+    n and L uniform in their ranges."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
+    Ls = rng.integers(L_range[0], L_range[1] + 1, size=count)
+    return [(int(ns[i]), int(Ls[i]), seed * 1000 + i) for i in range(count)]
+
+
 def founders(n=100000, L=5000, seed=3, nf=50, mut=0.01, miss=0.01):
     rng = np.random.default_rng(seed)
     F = rng.integers(1, nf + 1, size=(nf, L), dtype=np.int64)

@@ -171,6 +171,32 @@ int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int
  * no counterpart in the reference beyond tch_run's shape. */
 int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, char* err, int errlen);
 
+/* ------------------------------------------------------------------------ */
+/* Many independent datasets in one persistent launch (no counterpart in the */
+/* reference, which runs one dataset per process).                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct cgh_batch cgh_batch;
+
+/* Batch of `count` uploaded datasets (borrowed: they must outlive the batch)
+ * sharing one output arena: dataset d's upper-triangle stripe layout (stripe
+ * height S) starts at element cgh_batch_out_offset(b, d).  tile: 0 = chosen
+ * per dataset (256 for n >= 4096, else 128), or 128 / 256 for all.  All their
+ * tiles go into two device work queues (one per tile size), longest first. */
+int cgh_batch_create(int device, cgh_dataset* const* sets, int count, int S, int tile, cgh_batch** out,
+                     char* err, int errlen);
+size_t cgh_batch_out_elems(const cgh_batch* b);
+size_t cgh_batch_out_offset(const cgh_batch* b, int d);
+int cgh_batch_tile(const cgh_batch* b, int d);
+
+/* Enqueue the whole batch on `stream`: one persistent product launch per
+ * tile size pulling tiles from the queue, then one rare-pair launch over all
+ * datasets.  use_graph != 0 (with a non-NULL stream) captures that sequence
+ * into a HIP graph on the first call and replays it while d_out, stream and
+ * kernel stay the same.  d_out: cgh_batch_out_elems(b) u16, 16-byte aligned. */
+int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int kernel, int use_graph, char* err, int errlen);
+void cgh_batch_free(cgh_batch* b);
+
 /* Measured peaks of the units the kernels use, from register-only loops on
  * `device`: int8 MFMA (TOP/s), fp4 MFMA (TFLOP/s), VALU AND + popcount
  * (TOP/s, 2 ops per bit pair) and a streaming copy (GB/s, read + write).

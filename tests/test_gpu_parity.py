@@ -122,6 +122,50 @@ def test_dataset_stream(cgh, gpu, workloads, S, tau):
     assert np.array_equal(H, oracle.brute_full(X))
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_batch_matches_oracle_and_graph_replays(cgh, gpu, workloads, kernel):
+    """Many uneven datasets in one persistent launch: every dataset's layout
+    equals the oracle; HIP-graph replays (with eager work in between) repeat
+    the eager result exactly."""
+    shapes = [(1, 50, 1), (2, 40, 2), (300, 97, 3), (700, 410, 4), (1500, 64, 5), (5000, 300, 6), (2600, 900, 7)]
+    S = 512
+    sets, Xs = [], []
+    for n, L, seed in shapes:
+        calls = workloads.zipf_iid(n, L, seed)
+        if seed % 2:
+            calls = workloads.hashed_ids(calls, bits=64 if seed % 3 else 32, seed=seed)
+        X = cgh.dense_ranks(calls)
+        Xs.append(X)
+        tau = [1, 2, 8, 64, 4096, 32, 16][seed - 1]
+        sets.append(cgh.Dataset(cgh.Encoded(X, tau, 256)))
+    b = cgh.Batch(sets, S)
+    assert [b.tile(d) for d in range(len(shapes))] == [128, 128, 128, 128, 128, 256, 128]
+    out = gpu.full((b.out_elems(),), 0x3c3c, dtype=gpu.int16, device="cuda")
+    st = gpu.cuda.current_stream()
+    print(f"batch kernel={kernel} eager", flush=True)
+    b.run(out.data_ptr(), st.cuda_stream, kernel, graph=False)
+    gpu.cuda.synchronize()
+    def squares(flat_all):
+        return [cgh.upper_to_square(flat_all[b.offset(d): b.offset(d) + cgh.upper_elems(X.shape[0], S)],
+                                    X.shape[0], S) for d, X in enumerate(Xs)]
+
+    eager = squares(out.cpu().numpy().view(np.uint16))
+    for d, X in enumerate(Xs):
+        assert np.array_equal(eager[d], oracle.brute_full(X)), d
+    side = gpu.cuda.Stream()  # graphs are captured on a non-default stream
+    for rep in range(3):
+        out.fill_(0x1234)
+        gpu.cuda.synchronize()
+        print(f"batch kernel={kernel} graph replay {rep}", flush=True)
+        b.run(out.data_ptr(), side.cuda_stream, kernel, graph=True)
+        gpu.cuda.synchronize()
+        got = squares(out.cpu().numpy().view(np.uint16))
+        assert all(np.array_equal(x, y) for x, y in zip(got, eager)), rep
+    b.close()
+    for s in sets:
+        s.close()
+
+
 def test_rare_pass_many_chunks(cgh, gpu):
     """Rows longer than one LDS chunk (n > 32,768): chunk clipping of the
     partner ranges, checked on a row sample against the oracle."""
