@@ -30,7 +30,11 @@ EXPORTS = (
     "cgh_dataset_upload", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
     "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream",
     "cgh_batch_create", "cgh_batch_out_elems", "cgh_batch_out_offset", "cgh_batch_tile", "cgh_batch_run",
-    "cgh_batch_free", "cgh_measure_peaks", "cgh_dataset_tile",
+    "cgh_batch_free",
+    "cgh_codec_create", "cgh_codec_info", "cgh_codec_members", "cgh_codec_encode", "cgh_codec_free",
+    "cgh_db_create", "cgh_db_info", "cgh_db_free",
+    "cgh_server_create", "cgh_server_slots", "cgh_server_submit", "cgh_server_wait", "cgh_server_free",
+    "cgh_measure_peaks", "cgh_dataset_tile",
 )
 
 PHASE_PRODUCT, PHASE_RARE = 1, 2
@@ -91,6 +95,19 @@ def load():
         "cgh_batch_tile": (I, [P, I]),
         "cgh_batch_run": (I, [P, P, P, I, I, C.c_char_p, I]),
         "cgh_batch_free": (None, [P]),
+        "cgh_codec_create": (I, [P, I, I, I, C.POINTER(Encoding), C.POINTER(P), C.c_char_p, I]),
+        "cgh_codec_info": (None, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)] + [C.POINTER(S)] * 4),
+        "cgh_codec_members": (C.POINTER(C.c_uint32), [P]),
+        "cgh_codec_encode": (C.c_long, [P, P, I, P, P, P, P, S, C.c_char_p, I]),
+        "cgh_codec_free": (None, [P]),
+        "cgh_db_create": (I, [I, P, I, I, I, C.POINTER(P), C.c_char_p, I]),
+        "cgh_db_info": (I, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(S)]),
+        "cgh_db_free": (None, [P]),
+        "cgh_server_create": (I, [P, I, I, S, I, C.POINTER(P), C.c_char_p, I]),
+        "cgh_server_slots": (I, [P]),
+        "cgh_server_submit": (C.c_long, [P, P, I, C.c_char_p, I]),
+        "cgh_server_wait": (I, [P, C.c_long, P, P, P, C.POINTER(C.c_double), C.c_char_p, I]),
+        "cgh_server_free": (None, [P]),
         "cgh_dataset_tile": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
@@ -314,6 +331,113 @@ class Batch:
         err = _err()
         _check(load().cgh_batch_run(self.handle, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), kernel, int(graph), err,
                                     512), err)
+
+
+class QueryCodec:
+    """Host codec of a database (cgh_codec_*): query ids -> the database's
+    frequent columns and rare member ranges.  Host memory only."""
+
+    def __init__(self, calls: np.ndarray, tau: int, want_encoding: bool = False):
+        calls = np.ascontiguousarray(calls, dtype=np.uint64)
+        n, L = calls.shape
+        self.handle = C.c_void_p()
+        self.enc = Encoding() if want_encoding else None
+        err = _err()
+        _check(load().cgh_codec_create(_ptr(calls), n, L, int(tau), C.byref(self.enc) if want_encoding else None,
+                                       C.byref(self.handle), err, 512), err)
+        v = [C.c_int() for _ in range(3)] + [C.c_size_t() for _ in range(4)]
+        load().cgh_codec_info(self.handle, *[C.byref(x) for x in v])
+        self.n, self.L, self.tau, self.kw1, self.kw2, self.D, self.n_members = [x.value for x in v]
+
+    def members(self) -> np.ndarray:
+        if not self.n_members:
+            return np.zeros(0, dtype=np.uint32)
+        return np.ctypeslib.as_array(load().cgh_codec_members(self.handle), shape=(self.n_members,)).copy()
+
+    def encode(self, calls: np.ndarray):
+        """-> (O q x kw1 u32, P q x kw2 u32, rstart q+1 u32, rentry u64)"""
+        calls = np.ascontiguousarray(calls, dtype=np.uint64)
+        q = calls.shape[0]
+        O = np.full((q, self.kw1), 0xFFFFFFFF, dtype=np.uint32)  # overwritten rows
+        P = np.full((q, self.kw2), 0xFFFFFFFF, dtype=np.uint32)
+        rs = np.zeros(q + 1, dtype=np.uint32)
+        re = np.zeros(max(q * self.L, 1), dtype=np.uint64)
+        err = _err()
+        m = load().cgh_codec_encode(self.handle, _ptr(calls), q, _ptr(O), _ptr(P), _ptr(rs), _ptr(re), re.size,
+                                    err, 512)
+        if m < 0:
+            raise CghError(err.value.decode(errors="replace"))
+        return O, P, rs, re[:m].copy()
+
+    def close(self):
+        if _lib is not None and self.handle:
+            _lib.cgh_codec_free(self.handle)
+            self.handle = C.c_void_p()
+        if _lib is not None and self.enc is not None:
+            _lib.cgh_encoding_free(C.byref(self.enc))
+            self.enc = None
+
+    __del__ = close
+
+
+class Database:
+    """Resident database (cgh_db_*)."""
+
+    def __init__(self, calls: np.ndarray, tau: int, device: int = 0):
+        calls = np.ascontiguousarray(calls, dtype=np.uint64)
+        n, L = calls.shape
+        self.handle = C.c_void_p()
+        err = _err()
+        _check(load().cgh_db_create(device, _ptr(calls), n, L, int(tau), C.byref(self.handle), err, 512), err)
+        v = [C.c_int(), C.c_int(), C.c_int()]
+        d = C.c_size_t()
+        load().cgh_db_info(self.handle, *[C.byref(x) for x in v], C.byref(d))
+        self.n, self.L, self.tau, self.D = v[0].value, v[1].value, v[2].value, d.value
+
+    def close(self):
+        if _lib is not None and self.handle:
+            _lib.cgh_db_free(self.handle)
+            self.handle = C.c_void_p()
+
+    __del__ = close
+
+
+class Server:
+    """Continuous-batching query server over a Database (cgh_server_*)."""
+
+    def __init__(self, db: Database, slots: int = 4, max_batch: int = 1024, hbm_budget: int = 0, kernel: int = 0):
+        self.db = db
+        self.handle = C.c_void_p()
+        err = _err()
+        _check(load().cgh_server_create(db.handle, slots, max_batch, hbm_budget, kernel, C.byref(self.handle), err,
+                                        512), err)
+        self.slots = load().cgh_server_slots(self.handle)
+
+    def close(self):
+        if _lib is not None and self.handle:
+            _lib.cgh_server_free(self.handle)
+            self.handle = C.c_void_p()
+
+    __del__ = close
+
+    def submit(self, calls: np.ndarray) -> int:
+        calls = np.ascontiguousarray(calls, dtype=np.uint64)
+        err = _err()
+        t = load().cgh_server_submit(self.handle, _ptr(calls), calls.shape[0], err, 512)
+        if t < 0:
+            raise CghError(err.value.decode(errors="replace"))
+        return t
+
+    def wait(self, ticket: int, q: int, want_dist: bool = True):
+        """-> (dist q x n u16 or None, top_d q x 10, top_i q x 10, latency s)"""
+        dist = np.zeros((q, self.db.n), dtype=np.uint16) if want_dist else None
+        top_d = np.zeros((q, 10), dtype=np.uint16)
+        top_i = np.zeros((q, 10), dtype=np.uint32)
+        lat = C.c_double()
+        err = _err()
+        _check(load().cgh_server_wait(self.handle, ticket, _ptr(dist) if dist is not None else None, _ptr(top_d),
+                                      _ptr(top_i), C.byref(lat), err, 512), err)
+        return dist, top_d, top_i, lat.value
 
 
 def upper_to_square(flat: np.ndarray, n: int, S: int) -> np.ndarray:

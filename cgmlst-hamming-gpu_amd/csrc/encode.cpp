@@ -336,3 +336,181 @@ extern "C" int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, si
     }
     return best;
 }
+
+// ---------------------------------------------------------------------------
+// Query codec of a resident database
+// ---------------------------------------------------------------------------
+
+struct cgh_codec {
+    int n = 0, L = 0, tau = 0;
+    size_t kw1 = 0, kw2 = 0, D = 0;
+    std::vector<std::vector<uint64_t>> ids;  // locus -> sorted distinct ids; rank a <-> ids[k][a-1]
+    std::vector<size_t> base;                // locus -> first one-hot column
+    std::vector<std::vector<int32_t>> col;   // locus, rank -> local frequent column, or -1
+    std::vector<std::vector<int64_t>> lst;   // locus, rank -> rare list number, or -1
+    std::vector<uint32_t> lstart;            // list g = members[lstart[g], lstart[g+1])
+    std::vector<uint32_t> members;           // database rows per rare allele, ascending
+};
+
+extern "C" void cgh_codec_free(cgh_codec* c) { delete c; }
+
+extern "C" int cgh_codec_create(const uint64_t* calls, int n, int L, int tau, cgh_encoding* enc, cgh_codec** out,
+                                char* err, int errlen) {
+    *out = NULL;
+    if (n < 1 || L < 1 || L > 65535 || tau < 1) {
+        set_err(err, errlen, "cgh_codec_create: bad arguments (n=%d L=%d tau=%d)", n, L, tau);
+        return 1;
+    }
+    std::vector<uint16_t> X((size_t)n * L);
+    if (cgh_dense_ranks(calls, n, L, X.data(), err, errlen)) return 1;
+    cgh_codec* c = new cgh_codec();
+    c->n = n;
+    c->L = L;
+    c->tau = tau = std::min(tau, n + 1);
+    c->ids.resize((size_t)L);
+    c->col.resize((size_t)L);
+    c->lst.resize((size_t)L);
+    std::vector<std::vector<uint32_t>> cnt;
+    locus_counts(X.data(), n, L, cnt);
+    std::vector<size_t> nf((size_t)L, 0), nl((size_t)L, 0), nm((size_t)L, 0);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int k = 0; k < L; k++) {
+        std::vector<uint64_t>& u = c->ids[k];
+        u.reserve(cnt[k].size() - 1);
+        for (int i = 0; i < n; i++)
+            if (calls[(size_t)i * L + k]) u.push_back(calls[(size_t)i * L + k]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        c->col[k].assign(cnt[k].size(), -1);
+        c->lst[k].assign(cnt[k].size(), -1);
+        // frequent columns as in cgh_encode; every other allele (singletons
+        // included: a query may carry a database singleton) gets a list
+        for (size_t a = 1; a < cnt[k].size(); a++) {
+            if (cnt[k][a] >= (uint32_t)tau) {
+                c->col[k][a] = (int32_t)nf[k]++;
+            } else {
+                c->lst[k][a] = (int64_t)nl[k]++;
+                nm[k] += cnt[k][a];
+            }
+        }
+    }
+    c->base.assign((size_t)L + 1, 0);
+    std::vector<size_t> l0((size_t)L + 1, 0), m0((size_t)L + 1, 0);
+    for (int k = 0; k < L; k++) {
+        c->base[k + 1] = c->base[k] + nf[k];
+        l0[k + 1] = l0[k] + nl[k];
+        m0[k + 1] = m0[k] + nm[k];
+    }
+    if (m0[L] >= 0xffffffffull) {
+        delete c;
+        set_err(err, errlen, "database rare lists exceed 2^32 members: raise tau");
+        return 1;
+    }
+    c->D = c->base[L];
+    c->kw1 = up(std::max<size_t>(c->D, 1), 256) / 32;
+    c->kw2 = up((size_t)L, 256) / 32;
+    c->lstart.assign(l0[L] + 1, 0);
+    c->members.assign(m0[L], 0);
+#pragma omp parallel
+    {
+        std::vector<uint32_t> cur;
+#pragma omp for schedule(dynamic, 4)
+        for (int k = 0; k < L; k++) {
+            cur.assign(cnt[k].size(), 0);
+            size_t m = m0[k];
+            for (size_t a = 1; a < cnt[k].size(); a++)
+                if (c->lst[k][a] >= 0) {
+                    const int64_t g = c->lst[k][a] + (int64_t)l0[k];
+                    c->lst[k][a] = g;
+                    c->lstart[g] = (uint32_t)m;
+                    cur[a] = (uint32_t)m;
+                    m += cnt[k][a];
+                }
+            for (int i = 0; i < n; i++) {
+                const uint16_t a = X[(size_t)i * L + k];
+                if (a && c->lst[k][a] >= 0) c->members[cur[a]++] = (uint32_t)i;
+            }
+        }
+    }
+    c->lstart[l0[L]] = (uint32_t)m0[L];
+    if (enc) {
+        if (cgh_encode(X.data(), n, L, tau, 256, enc, err, errlen)) {
+            delete c;
+            return 1;
+        }
+        if (enc->kw1 != c->kw1 || enc->kw2 != c->kw2 || enc->D != c->D) {  // same column map by construction
+            cgh_encoding_free(enc);
+            delete c;
+            set_err(err, errlen, "cgh_codec_create: column map disagrees with cgh_encode");
+            return 1;
+        }
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" void cgh_codec_info(const cgh_codec* c, int* n, int* L, int* tau, size_t* kw1, size_t* kw2, size_t* D,
+                               size_t* n_members) {
+    if (n) *n = c->n;
+    if (L) *L = c->L;
+    if (tau) *tau = c->tau;
+    if (kw1) *kw1 = c->kw1;
+    if (kw2) *kw2 = c->kw2;
+    if (D) *D = c->D;
+    if (n_members) *n_members = c->members.size();
+}
+
+extern "C" const uint32_t* cgh_codec_members(const cgh_codec* c) { return c->members.data(); }
+
+extern "C" long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int q, uint32_t* O, uint32_t* P,
+                                 uint32_t* rstart, uint64_t* rentry, size_t cap, char* err, int errlen) {
+    const int L = c->L;
+    if (q < 0 || cap < (size_t)q * (size_t)L) {
+        set_err(err, errlen, "cgh_codec_encode: %d queries need room for %zu rare entries (have %zu)", q,
+                (size_t)q * (size_t)L, cap);
+        return -1;
+    }
+    // row r writes its entries from slot r * L on; compacted below
+    std::atomic<int> bad{0};
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < q; r++) {
+        uint32_t* orow = O + (size_t)r * c->kw1;
+        uint32_t* prow = P + (size_t)r * c->kw2;
+        memset(orow, 0, c->kw1 * 4);
+        memset(prow, 0, c->kw2 * 4);
+        uint64_t* re = rentry + (size_t)r * L;
+        uint32_t m = 0;
+        for (int k = 0; k < L; k++) {
+            const uint64_t v = calls[(size_t)r * L + k];
+            if (!v) continue;
+            prow[k >> 5] |= 1u << (k & 31);
+            const std::vector<uint64_t>& u = c->ids[k];
+            const auto it = std::lower_bound(u.begin(), u.end(), v);
+            if (it == u.end() || *it != v) continue;  // an id the database never saw matches no row
+            const size_t a = (size_t)(it - u.begin()) + 1;
+            if (c->col[k][a] >= 0) {
+                const size_t cc = c->base[k] + (size_t)c->col[k][a];
+                orow[cc >> 5] |= 1u << (cc & 31);
+            } else {
+                const size_t g = (size_t)c->lst[k][a];
+                const uint32_t b = c->lstart[g], e = c->lstart[g + 1];
+                if (b > e || e > c->members.size()) bad = 1;  // consumers read members[b, e)
+                re[m++] = ((uint64_t)e << 32) | b;
+            }
+        }
+        rstart[r + 1] = m;  // count for now
+    }
+    if (bad) {
+        set_err(err, errlen, "cgh_codec_encode: rare list outside the member array");
+        return -1;
+    }
+    size_t tot = 0;
+    rstart[0] = 0;
+    for (int r = 0; r < q; r++) {
+        const uint32_t m = rstart[r + 1];
+        if (tot != (size_t)r * L) memmove(rentry + tot, rentry + (size_t)r * L, (size_t)m * 8);
+        tot += m;
+        rstart[r + 1] = (uint32_t)tot;
+    }
+    return (long)tot;
+}

@@ -27,7 +27,14 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "cgmlst_hamming.h"
@@ -56,6 +63,12 @@ struct TileJob {
     int r0;             // first row of the layout
     int S;              // stripe height
     uint16_t* out;      // stripe layout base
+    // Rectangular form (rect != 0): rows of (O, P) against the nB rows of
+    // (OB, PB) in the same column space; out is n x nB row-major, every cell
+    // kept.  The symmetric form has OB = O, PB = P, nB = n.
+    const uint32_t* OB;
+    const uint32_t* PB;
+    int nB, rect;
 };
 
 struct TileArgs {
@@ -161,11 +174,12 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
 
     const int tid = threadIdx.x;
     const int sr0 = a.r0 + s * a.S;              // first row of the stripe
-    const int rows = min(a.S, a.n - sr0), w = a.n - sr0;
-    uint16_t* __restrict__ out = a.out + stripe_off((size_t)(a.n - a.r0), (size_t)a.S, (size_t)s);
+    const int rows = min(a.S, a.n - sr0), w = a.rect ? a.nB : a.n - sr0;
+    uint16_t* __restrict__ out = a.rect ? a.out + (size_t)(sr0 - a.r0) * a.nB
+                                        : a.out + stripe_off((size_t)(a.n - a.r0), (size_t)a.S, (size_t)s);
     const int rt0 = I * BM, ct0 = J * BM;  // tile origin, stripe-relative
 
-    if (J < I) {  // entirely below the diagonal: every cell has j < i
+    if (!a.rect && J < I) {  // entirely below the diagonal: every cell has j < i
         const int rend = min(BM, rows - rt0), cend = min(BM, w - ct0);
         for (int e = tid; e < rend * BM; e += NT) {
             const int r = e / BM, c = e % BM;
@@ -179,7 +193,7 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     const int l32 = lane & 31, g = lane >> 5;
 
     const int nc1 = a.kw1 / KC, nc = nc1 + a.kw2 / KC;
-    const int growA = sr0 + rt0, growB = sr0 + ct0;
+    const int growA = sr0 + rt0, growB = a.rect ? ct0 : sr0 + ct0;
 
     // Slab c (KC words of every A and B row of the tile) -> ring stage st,
     // straight from global memory into LDS (global_load_lds_dwordx4: lane l
@@ -187,18 +201,18 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     // n read row n-1: their products only reach output cells that are never
     // stored.
     auto issue_slab = [&](int c, int st) {
-        const uint32_t* base;
+        const uint32_t *baseA, *baseB;
         int kw, k0;
-        if (c < nc1) { base = a.O; kw = a.kw1; k0 = c * KC; }
-        else { base = a.P; kw = a.kw2; k0 = (c - nc1) * KC; }
+        if (c < nc1) { baseA = a.O; baseB = a.OB; kw = a.kw1; k0 = c * KC; }
+        else { baseA = a.P; baseB = a.PB; kw = a.kw2; k0 = (c - nc1) * KC; }
 #pragma unroll
         for (int u = 0; u < LPW; u++) {
             const int v = wave * LPW + u;
             const int side = v / (KG * (BM / 64));
             const int rem = v % (KG * (BM / 64));
             const int grp = rem / (BM / 64), rb = rem % (BM / 64);
-            const int grow = min((side ? growB : growA) + rb * 64 + lane, a.n - 1);
-            const uint32_t* src = base + (size_t)grow * kw + k0 + grp * 4;
+            const int grow = side ? min(growB + rb * 64 + lane, a.nB - 1) : min(growA + rb * 64 + lane, a.n - 1);
+            const uint32_t* src = (side ? baseB : baseA) + (size_t)grow * kw + k0 + grp * 4;
             // LDS base of this wavefront's 64 rows goes to M0.  Written as an
             // instruction rather than the builtin so that the compiler does
             // not wait for every LDS-DMA transfer before each LDS read (it
@@ -347,7 +361,7 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int r = rt0 + wm0 + i * 32 + (e >> 2) * 8 + g * 4 + (e & 3);
-                if (r < rows) out[(size_t)r * w + c] = (uint16_t)(c > r ? -(int)acc[i][j][e] : 0);
+                if (r < rows) out[(size_t)r * w + c] = (uint16_t)(a.rect || c > r ? -(int)acc[i][j][e] : 0);
             }
         }
 }
@@ -375,6 +389,29 @@ mfma_tile(TileArgs a) {
     tile_body<BM, WM, WN, FP4>(a.job, s, (int)(local / tn), (int)(local % tn), ring);
 }
 
+// A job read from memory by a uniform index is uniform: say so, so that its
+// fields live in scalar registers.
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    const uint64_t v = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T*)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ void scalarize(TileJob& j) {
+    j.O = uniform_ptr(j.O);
+    j.P = uniform_ptr(j.P);
+    j.OB = uniform_ptr(j.OB);
+    j.PB = uniform_ptr(j.PB);
+    j.out = uniform_ptr(j.out);
+    j.kw1 = __builtin_amdgcn_readfirstlane(j.kw1);
+    j.kw2 = __builtin_amdgcn_readfirstlane(j.kw2);
+    j.n = __builtin_amdgcn_readfirstlane(j.n);
+    j.r0 = __builtin_amdgcn_readfirstlane(j.r0);
+    j.S = __builtin_amdgcn_readfirstlane(j.S);
+    j.nB = __builtin_amdgcn_readfirstlane(j.nB);
+    j.rect = __builtin_amdgcn_readfirstlane(j.rect);
+}
+
 // Persistent form for many datasets: each workgroup pulls tiles (job,
 // stripe, I, J) from a device work queue until the list is exhausted; the
 // list is ordered longest-first, so uneven datasets finish together.
@@ -387,12 +424,13 @@ mfma_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* 
     for (;;) {
         if (threadIdx.x == 0) slot = atomicAdd(next, 1);
         __syncthreads();
-        const int t = slot;
+        const int t = __builtin_amdgcn_readfirstlane(slot);  // uniform: tile and job load as scalars
         __syncthreads();  // every wavefront has read slot, and left the previous tile
         if (t >= T) break;
         const int4 tl = tiles[t];
         TileJob jb = jobs[tl.x];
         jb.out = base + outoff[tl.x];
+        scalarize(jb);
         tile_body<BM, WM, WN, FP4>(jb, tl.y, tl.z, tl.w, ring);
     }
 }
@@ -440,6 +478,7 @@ struct RareJob {
     const uint32_t* gmem;      // group members
     long long block0;          // first workgroup of this job in a batched launch
     int d;                     // dataset of this job in a batched launch
+    int rect, nB;              // rectangular form: rows against nB columns
 };
 
 struct RareShared {
@@ -463,12 +502,16 @@ __device__ __forceinline__ void rare_rows_body(const RareJob& jb, int i, int ch,
     uint32_t* cells = reinterpret_cast<uint32_t*>(cells4);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
-    const int j0 = i + 1 + ch * RCW, j1 = min(n, j0 + RCW);
-    if (e0 == e1 || j0 >= n) return;
-    // out holds the stripe layout of rows r0.. (stripe s = rows r0+s*S ..);
-    // u16 index of cell (i, j) is base + j
-    const size_t s = (size_t)((i - r0) / S), sr0 = (size_t)r0 + s * S;
-    const size_t base = stripe_off((size_t)(n - r0), (size_t)S, s) + ((size_t)i - sr0) * ((size_t)n - sr0) - sr0;
+    // symmetric: cells j in (i, n) of the stripe layout of rows r0..
+    // (stripe s = rows r0+s*S ..); rectangular: cells j in [0, nB) of row
+    // i - r0 of an (n - r0) x nB row-major matrix.  u16 index of (i, j) is
+    // base + j.
+    const int jlo = jb.rect ? 0 : i + 1, jhi = jb.rect ? jb.nB : n;
+    const int j0 = jlo + ch * RCW, j1 = min(jhi, j0 + RCW);
+    if (e0 == e1 || j0 >= jhi) return;
+    const size_t s = jb.rect ? 0 : (size_t)((i - r0) / S), sr0 = (size_t)r0 + s * S;
+    const size_t base = jb.rect ? (size_t)(i - r0) * (size_t)jb.nB
+                                : stripe_off((size_t)(n - r0), (size_t)S, s) + ((size_t)i - sr0) * ((size_t)n - sr0) - sr0;
     const size_t g0 = base + j0, g1 = base + j1;   // cells [g0, g1)
     const size_t a0 = g0 & ~(size_t)7;              // first 16-byte word touched
     const int nq = (int)((g1 - a0 + 7) >> 3);       // 16-byte words staged
@@ -635,6 +678,7 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
     TileArgs a;
     a.job.O = O; a.job.P = P; a.job.kw1 = kw1; a.job.kw2 = kw2; a.job.n = n; a.job.r0 = r0; a.job.S = S;
     a.job.out = out;
+    a.job.OB = O; a.job.PB = P; a.job.nB = n; a.job.rect = 0;
     a.ns = ns;
     a.nbc = (n - r0 + tile - 1) / tile;
     a.q = S / tile;
@@ -961,7 +1005,7 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         b->tile.push_back(tl);
         b->offs.push_back(b->total);
         b->total += cgh_upper_elems(ds->n, S);
-        jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL};
+        jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL, ds->dO, ds->dP, ds->n, 0};
         const int k = tl == 256 ? 0 : 1;
         const long long cost = (long long)(ds->kw1 + ds->kw2);
         for (int s = 0, r0 = 0; r0 < ds->n; s++, r0 += S) {
@@ -1085,6 +1129,432 @@ extern "C" int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int ke
 fail:
     if (graph) (void)hipGraphDestroy(graph);
     return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Resident database and a continuous-batching query server
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int TOPK = 10;
+
+// k nearest database rows of each query row of dist (rows x nB u16):
+// ascending (distance, row index).  One workgroup per query: every thread
+// keeps a sorted list of its best TOPK, the lists meet in LDS and wavefront 0
+// draws the TOPK overall minima one by one.
+__global__ void __launch_bounds__(256)
+top_k_rows(const uint16_t* __restrict__ dist, int nB, uint16_t* __restrict__ top_d, uint32_t* __restrict__ top_i) {
+    __shared__ uint64_t cand[256 * TOPK];
+    const int q = blockIdx.x, tid = threadIdx.x;
+    const uint16_t* row = dist + (size_t)q * nB;
+    uint64_t best[TOPK];
+#pragma unroll
+    for (int k = 0; k < TOPK; k++) best[k] = ~0ull;
+    for (int j = tid; j < nB; j += 256) {
+        const uint64_t key = ((uint64_t)row[j] << 32) | (uint32_t)j;
+        if (key < best[TOPK - 1]) {
+            int k = TOPK - 1;
+            while (k > 0 && best[k - 1] > key) {
+                best[k] = best[k - 1];
+                k--;
+            }
+            best[k] = key;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < TOPK; k++) cand[tid * TOPK + k] = best[k];
+    __syncthreads();
+    if (tid >= 64) return;
+    // lane l owns candidates [l*40, l*40+40) (sorted runs of 10): merge by
+    // repeated wavefront minimum
+    int head[4] = {0, 0, 0, 0};
+    for (int out = 0; out < TOPK; out++) {
+        uint64_t mine = ~0ull;
+        int which = -1;
+        for (int r = 0; r < 4; r++) {
+            const int t = tid * 4 + r;
+            if (head[r] < TOPK && cand[t * TOPK + head[r]] < mine) {
+                mine = cand[t * TOPK + head[r]];
+                which = r;
+            }
+        }
+        uint64_t m = mine;
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t o = __shfl_xor(m, off, 64);
+            m = o < m ? o : m;
+        }
+        if (which >= 0 && mine == m) head[which]++;  // keys are unique (row index)
+        if (tid == 0) {
+            top_d[(size_t)q * TOPK + out] = m == ~0ull ? 0xffff : (uint16_t)(m >> 32);
+            top_i[(size_t)q * TOPK + out] = m == ~0ull ? 0xffffffffu : (uint32_t)m;
+        }
+    }
+}
+
+}  // namespace
+
+struct cgh_db {
+    int device = 0, n = 0, L = 0, tau = 0;
+    size_t kw1 = 0, kw2 = 0, D = 0;
+    cgh_codec* codec = NULL;    // host dictionaries: query ids -> database columns and rare lists
+    cgh_dataset* ds = NULL;     // O, P of the database on the device
+    uint32_t* dqmem = NULL;     // the codec's member array on the device
+};
+
+extern "C" void cgh_db_free(cgh_db* db) {
+    if (!db) return;
+    (void)hipSetDevice(db->device);
+    if (db->ds) cgh_dataset_free(db->ds);
+    if (db->dqmem) (void)hipFree(db->dqmem);
+    if (db->codec) cgh_codec_free(db->codec);
+    delete db;
+}
+
+extern "C" int cgh_db_create(int device, const uint64_t* calls, int n, int L, int tau, cgh_db** out, char* err,
+                             int errlen) {
+    cgh_db* db = new cgh_db();
+    cgh_encoding enc;
+    size_t nmem = 0;
+    memset(&enc, 0, sizeof enc);
+    *out = NULL;
+    db->device = device;
+    if (cgh_codec_create(calls, n, L, tau, &enc, &db->codec, err, errlen)) goto fail;
+    cgh_codec_info(db->codec, &db->n, &db->L, &db->tau, &db->kw1, &db->kw2, &db->D, &nmem);
+    if (cgh_dataset_upload(device, &enc, &db->ds, err, errlen)) goto fail;
+    cgh_encoding_free(&enc);
+    HK(hipSetDevice(device));
+    HK(hipMalloc(&db->dqmem, std::max<size_t>(nmem, 1) * 4));
+    if (nmem) HK(hipMemcpy(db->dqmem, cgh_codec_members(db->codec), nmem * 4, hipMemcpyHostToDevice));
+    *out = db;
+    return 0;
+fail:
+    cgh_encoding_free(&enc);
+    cgh_db_free(db);
+    return 1;
+}
+
+extern "C" int cgh_db_info(const cgh_db* db, int* n, int* L, int* tau, size_t* D) {
+    if (n) *n = db->n;
+    if (L) *L = db->L;
+    if (tau) *tau = db->tau;
+    if (D) *D = db->D;
+    return 0;
+}
+
+struct cgh_server {
+    struct Request {
+        long id;
+        int q;
+        std::vector<uint64_t> calls;
+        double t_submit = 0, t_done = 0;
+        std::vector<uint16_t> dist, top_d;
+        std::vector<uint32_t> top_i;
+        bool done = false;
+        std::string err;
+    };
+    struct Slot {
+        hipStream_t st = NULL;
+        hipEvent_t ev = NULL;
+        bool busy = false;
+        std::vector<Request*> reqs;
+        int q = 0;
+        size_t bytes = 0;
+        uint32_t *dO = NULL, *dP = NULL, *drs = NULL;
+        uint64_t* dre = NULL;
+        uint16_t *ddist = NULL, *dtopd = NULL;
+        uint32_t* dtopi = NULL;
+        uint32_t *hO = NULL, *hP = NULL, *hrs = NULL;
+        uint64_t* hre = NULL;
+        uint16_t *hdist = NULL, *htopd = NULL;
+        uint32_t* htopi = NULL;
+        size_t re_cap = 0;
+    };
+    cgh_db* db = NULL;
+    int max_batch = 0, nslots = 0, kernel = 0;
+    size_t budget = 0, inflight = 0;
+    std::vector<Slot> slots;
+    std::deque<Request*> pending;
+    std::unordered_map<long, Request*> reqs;
+    long next_id = 0;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::thread worker;
+    bool stop = false;
+    std::string fatal;
+};
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+size_t batch_bytes(const cgh_db* db, int q) {
+    return (size_t)q * (size_t)db->n * 2 + (size_t)up((size_t)q, 256) * (db->kw1 + db->kw2) * 4 +
+           (size_t)q * (size_t)db->L * 8 + (size_t)q * TOPK * 6;
+}
+
+// Enqueue one batch of queries on its slot: upload, product, rare pairs,
+// top-k, download.
+hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
+    cgh_db* db = sv->db;
+    const int q = sl.q, q_pad = (int)up((size_t)q, 256);
+    // queries in the database's columns, request by request into the
+    // slot's pinned buffers; rows q..q_pad-1 stay zero
+    size_t row = 0, ne = 0;
+    for (auto* r : sl.reqs) {
+        const long m = cgh_codec_encode(db->codec, r->calls.data(), r->q, sl.hO + row * db->kw1,
+                                        sl.hP + row * db->kw2, sl.hrs + row, sl.hre + ne, sl.re_cap - ne, NULL, 0);
+        if (m < 0) return hipErrorInvalidValue;
+        for (int k = 0; k <= r->q; k++) sl.hrs[row + k] += (uint32_t)ne;
+        row += (size_t)r->q;
+        ne += (size_t)m;
+    }
+    memset(sl.hO + (size_t)q * db->kw1, 0, (size_t)(q_pad - q) * db->kw1 * 4);
+    memset(sl.hP + (size_t)q * db->kw2, 0, (size_t)(q_pad - q) * db->kw2 * 4);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(sl.dO, sl.hO, (size_t)q_pad * db->kw1 * 4, hipMemcpyHostToDevice, sl.st))) return e;
+    if ((e = hipMemcpyAsync(sl.dP, sl.hP, (size_t)q_pad * db->kw2 * 4, hipMemcpyHostToDevice, sl.st))) return e;
+    if ((e = hipMemcpyAsync(sl.drs, sl.hrs, (size_t)(q + 1) * 4, hipMemcpyHostToDevice, sl.st))) return e;
+    if (ne && (e = hipMemcpyAsync(sl.dre, sl.hre, ne * 8, hipMemcpyHostToDevice, sl.st))) return e;
+    const int tile = q > 128 ? 256 : 128;
+    TileArgs a;
+    a.job = {sl.dO, sl.dP, (int)db->kw1, (int)db->kw2, q, 0, q, sl.ddist, db->ds->dO, db->ds->dP, db->n, 1};
+    a.ns = 1;
+    a.q = 0;
+    a.nbc = (db->n + tile - 1) / tile;
+    a.T = (long long)((q + tile - 1) / tile) * a.nbc;
+    a.chunk = (a.T + 7) / 8;
+    const bool fp4 = resolve_kernel(sv->kernel) == CGH_KERNEL_MFMA_FP4;
+    const dim3 grid((unsigned)(a.chunk * 8));
+    if (tile == 256 && fp4) hipLaunchKernelGGL((mfma_tile<256, 128, 128, true>), grid, dim3(256), 0, sl.st, a);
+    else if (tile == 256) hipLaunchKernelGGL((mfma_tile<256, 128, 128, false>), grid, dim3(256), 0, sl.st, a);
+    else if (fp4) hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, sl.st, a);
+    else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);
+    if (ne) {
+        const int chunks = (db->n + RCW - 1) / RCW;
+        const RareJob jb = {sl.ddist, q, 0, q, chunks, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n};
+        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)q * chunks)), dim3(RNT), 0, sl.st, jb);
+    }
+    hipLaunchKernelGGL(top_k_rows, dim3((unsigned)q), dim3(256), 0, sl.st, sl.ddist, db->n, sl.dtopd, sl.dtopi);
+    if ((e = hipGetLastError())) return e;
+    if ((e = hipMemcpyAsync(sl.hdist, sl.ddist, (size_t)q * db->n * 2, hipMemcpyDeviceToHost, sl.st))) return e;
+    if ((e = hipMemcpyAsync(sl.htopd, sl.dtopd, (size_t)q * TOPK * 2, hipMemcpyDeviceToHost, sl.st))) return e;
+    if ((e = hipMemcpyAsync(sl.htopi, sl.dtopi, (size_t)q * TOPK * 4, hipMemcpyDeviceToHost, sl.st))) return e;
+    return hipEventRecord(sl.ev, sl.st);
+}
+
+// Worker: retire finished batches, then give every free slot the longest
+// run of waiting requests (arrival order) that fits max_batch queries and the
+// HBM budget left by the batches in flight.
+void serve_loop(cgh_server* sv) {
+    (void)hipSetDevice(sv->db->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (!sv->stop) {
+        bool progressed = false;
+        for (auto& sl : sv->slots) {
+            if (!sl.busy) continue;
+            const hipError_t qe = hipEventQuery(sl.ev);
+            if (qe == hipErrorNotReady) continue;
+            size_t row = 0;
+            const double t = now_s();
+            for (auto* r : sl.reqs) {
+                if (qe != hipSuccess) {
+                    r->err = hipGetErrorString(qe);
+                } else {
+                    r->dist.assign(sl.hdist + row * sv->db->n, sl.hdist + (row + r->q) * sv->db->n);
+                    r->top_d.assign(sl.htopd + row * TOPK, sl.htopd + (row + r->q) * TOPK);
+                    r->top_i.assign(sl.htopi + row * TOPK, sl.htopi + (row + r->q) * TOPK);
+                }
+                row += (size_t)r->q;
+                r->t_done = t;
+                r->done = true;
+            }
+            sl.reqs.clear();
+            sl.busy = false;
+            sv->inflight -= sl.bytes;
+            progressed = true;
+        }
+        for (auto& sl : sv->slots) {
+            if (sl.busy || sv->pending.empty()) continue;
+            int q = 0;
+            while (!sv->pending.empty()) {
+                cgh_server::Request* r = sv->pending.front();
+                const int nq = q + r->q;
+                if (nq > sv->max_batch) break;
+                if (sv->inflight + batch_bytes(sv->db, nq) > sv->budget) break;
+                sl.reqs.push_back(r);
+                sv->pending.pop_front();
+                q = nq;
+            }
+            if (!q) break;  // the head waits for memory to free up
+            sl.q = q;
+            sl.bytes = batch_bytes(sv->db, q);
+            sv->inflight += sl.bytes;
+            sl.busy = true;
+            const hipError_t e = launch_batch(sv, sl);
+            if (e != hipSuccess) {
+                for (auto* r : sl.reqs) {
+                    r->err = hipGetErrorString(e);
+                    r->t_done = now_s();
+                    r->done = true;
+                }
+                sl.reqs.clear();
+                sl.busy = false;
+                sv->inflight -= sl.bytes;
+            }
+            progressed = true;
+        }
+        if (progressed) sv->cv.notify_all();
+        bool any_busy = false;
+        for (auto& sl : sv->slots) any_busy |= sl.busy;
+        if (any_busy) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+            lk.lock();
+        } else {
+            sv->cv.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" void cgh_server_free(cgh_server* sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    (void)hipSetDevice(sv->db->device);
+    for (auto& sl : sv->slots) {
+        if (sl.st) (void)hipStreamSynchronize(sl.st);
+        void* dev[] = {sl.dO, sl.dP, sl.drs, sl.dre, sl.ddist, sl.dtopd, sl.dtopi};
+        for (void* p : dev)
+            if (p) (void)hipFree(p);
+        void* host[] = {sl.hO, sl.hP, sl.hrs, sl.hre, sl.hdist, sl.htopd, sl.htopi};
+        for (void* p : host)
+            if (p) (void)hipHostFree(p);
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.st) (void)hipStreamDestroy(sl.st);
+    }
+    for (auto& kv : sv->reqs) delete kv.second;
+    delete sv;
+}
+
+extern "C" int cgh_server_create(cgh_db* db, int slots, int max_batch, size_t hbm_budget, int kernel,
+                                 cgh_server** out, char* err, int errlen) {
+    cgh_server* sv = new cgh_server();
+    size_t freeb = 0, totb = 0;
+    *out = NULL;
+    sv->db = db;
+    sv->max_batch = max_batch;
+    sv->kernel = kernel;
+    if (slots < 1 || max_batch < 1 || !valid_kernel(kernel)) {
+        set_err(err, errlen, "cgh_server_create: bad arguments (slots=%d max_batch=%d kernel=%d)", slots, max_batch,
+                kernel);
+        delete sv;
+        return 1;
+    }
+    HK(hipSetDevice(db->device));
+    HK(hipMemGetInfo(&freeb, &totb));
+    {
+        const char* env = getenv("CGH_FREE_BYTES");
+        if (env && *env) freeb = std::min(freeb, (size_t)strtoull(env, NULL, 10));
+    }
+    // admission budget: what the caller allows, within the free HBM left
+    // after the slot buffers (each slot holds one batch of up to max_batch)
+    {
+        const size_t per = batch_bytes(db, max_batch) + 4096;
+        const int fit = (int)std::min<size_t>((size_t)slots, freeb / per);
+        if (fit < 1) {
+            set_err(err, errlen, "no query batch of %d fits in %.2f GB of free device memory", max_batch, freeb / 1e9);
+            goto fail;
+        }
+        sv->nslots = fit;
+        sv->budget = hbm_budget ? std::min(hbm_budget, per * (size_t)fit) : per * (size_t)fit;
+    }
+    sv->slots.resize((size_t)sv->nslots);
+    for (auto& sl : sv->slots) {
+        const size_t qp = up((size_t)max_batch, 256);
+        sl.re_cap = (size_t)max_batch * db->L;
+        HK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+        HK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+        HK(hipMalloc(&sl.dO, qp * db->kw1 * 4));
+        HK(hipMalloc(&sl.dP, qp * db->kw2 * 4));
+        HK(hipMalloc(&sl.drs, ((size_t)max_batch + 1) * 4));
+        HK(hipMalloc(&sl.dre, std::max<size_t>(sl.re_cap, 1) * 8));
+        HK(hipMalloc(&sl.ddist, up((size_t)max_batch * db->n, 8) * 2));
+        HK(hipMalloc(&sl.dtopd, (size_t)max_batch * TOPK * 2));
+        HK(hipMalloc(&sl.dtopi, (size_t)max_batch * TOPK * 4));
+        HK(hipHostMalloc(&sl.hO, qp * db->kw1 * 4, hipHostMallocDefault));
+        HK(hipHostMalloc(&sl.hP, qp * db->kw2 * 4, hipHostMallocDefault));
+        HK(hipHostMalloc(&sl.hrs, ((size_t)max_batch + 1) * 4, hipHostMallocDefault));
+        HK(hipHostMalloc(&sl.hre, std::max<size_t>(sl.re_cap, 1) * 8, hipHostMallocDefault));
+        HK(hipHostMalloc(&sl.hdist, (size_t)max_batch * db->n * 2, hipHostMallocDefault));
+        HK(hipHostMalloc(&sl.htopd, (size_t)max_batch * TOPK * 2, hipHostMallocDefault));
+        HK(hipHostMalloc(&sl.htopi, (size_t)max_batch * TOPK * 4, hipHostMallocDefault));
+    }
+    sv->worker = std::thread(serve_loop, sv);
+    *out = sv;
+    return 0;
+fail:
+    cgh_server_free(sv);
+    return 1;
+}
+
+extern "C" int cgh_server_slots(const cgh_server* sv) { return sv->nslots; }
+
+extern "C" long cgh_server_submit(cgh_server* sv, const uint64_t* calls, int q, char* err, int errlen) {
+    if (q < 1 || q > sv->max_batch) {
+        set_err(err, errlen, "request of %d queries (1..%d accepted)", q, sv->max_batch);
+        return -1;
+    }
+    if (batch_bytes(sv->db, q) > sv->budget) {
+        set_err(err, errlen, "request of %d queries needs %.2f GB, over the %.2f GB admission budget", q,
+                batch_bytes(sv->db, q) / 1e9, sv->budget / 1e9);
+        return -1;
+    }
+    cgh_server::Request* r = new cgh_server::Request();
+    r->q = q;
+    r->calls.assign(calls, calls + (size_t)q * sv->db->L);
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        r->id = sv->next_id++;
+        r->t_submit = now_s();
+        sv->reqs[r->id] = r;
+        sv->pending.push_back(r);
+    }
+    sv->cv.notify_all();
+    return r->id;
+}
+
+extern "C" int cgh_server_wait(cgh_server* sv, long ticket, uint16_t* dist, uint16_t* top_d, uint32_t* top_i,
+                               double* latency_s, char* err, int errlen) {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) {
+        set_err(err, errlen, "unknown ticket %ld", ticket);
+        return 1;
+    }
+    cgh_server::Request* r = it->second;
+    sv->cv.wait(lk, [&] { return r->done; });
+    sv->reqs.erase(it);
+    lk.unlock();
+    const int rc = r->err.empty() ? 0 : 1;
+    if (rc) {
+        set_err(err, errlen, "GPU: %s", r->err.c_str());
+    } else {
+        if (dist) memcpy(dist, r->dist.data(), r->dist.size() * 2);
+        if (top_d) memcpy(top_d, r->top_d.data(), r->top_d.size() * 2);
+        if (top_i) memcpy(top_i, r->top_i.data(), r->top_i.size() * 4);
+        if (latency_s) *latency_s = r->t_done - r->t_submit;
+    }
+    delete r;
+    return rc;
 }
 
 // Stripes streamed to the host: the reference pipeline's shape (tch.cu:82-170,

@@ -107,3 +107,16 @@ def hashed_ids(calls, bits=64, seed=7):
             break
     table[uniq == 0] = 0
     return table[inv].reshape(calls.shape)
+
+
+def serving_sets(n_db, L, n_queries, seed, fresh=0.03, a_min=3, a_max=600):
+    """A database and a query set drawn from one long-tail population; a
+    fraction `fresh` of the query calls are ids the database never saw, and
+    all ids are arbitrary 64-bit values.  -> (db_calls, query_calls) u64."""
+    calls = zipf_iid(n_db + n_queries, L, seed, s=1.2, a_min=a_min, a_max=a_max)
+    rng = np.random.default_rng(seed)
+    q = calls[n_db:]
+    new = rng.random(q.shape) < fresh
+    q[new] = (10**12 + np.arange(int(new.sum()))).astype(np.uint64)
+    calls = hashed_ids(calls, bits=64, seed=seed)
+    return calls[:n_db], calls[n_db:]

@@ -197,6 +197,70 @@ int cgh_batch_tile(const cgh_batch* b, int d);
 int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int kernel, int use_graph, char* err, int errlen);
 void cgh_batch_free(cgh_batch* b);
 
+/* ------------------------------------------------------------------------ */
+/* Resident database + continuous-batching query server (no counterpart in   */
+/* the reference).                                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct cgh_codec cgh_codec;
+typedef struct cgh_db cgh_db;
+typedef struct cgh_server cgh_server;
+
+/* Host codec of a database of n profiles x L loci (u64 allele ids, 0 =
+ * missing) at threshold tau (clamped to n + 1): per-locus id dictionaries,
+ * the frequent-column map of cgh_encode (k_align 256) and, for every other
+ * allele (count < tau, singletons included), the ascending list of database
+ * rows that carry it, all lists concatenated into one member array.  enc (may
+ * be NULL) receives the database's own cgh_encode output. */
+int cgh_codec_create(const uint64_t* calls, int n, int L, int tau, cgh_encoding* enc, cgh_codec** out, char* err,
+                     int errlen);
+void cgh_codec_info(const cgh_codec* c, int* n, int* L, int* tau, size_t* kw1, size_t* kw2, size_t* D,
+                    size_t* n_members);
+const uint32_t* cgh_codec_members(const cgh_codec* c);
+
+/* Queries (q x L u64 ids) in the database's columns: rows 0..q-1 of O (kw1
+ * words each) and P (kw2 words each) are overwritten; rstart[0..q] indexes
+ * rentry, whose entries (end << 32 | begin) are member ranges of the rare
+ * alleles each query shares with the database (ids the database never saw
+ * match no row).  rentry needs room for q * L entries.  Returns the number
+ * of entries, or -1 with err set.  Then for query r and database row j:
+ * H = |P_r & P_j| - |O_r & O_j| - #{ranges of r holding j}. */
+long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int q, uint32_t* O, uint32_t* P, uint32_t* rstart,
+                      uint64_t* rentry, size_t cap, char* err, int errlen);
+void cgh_codec_free(cgh_codec* c);
+
+/* Database of n profiles x L loci (u64 allele ids, 0 = missing) resident on
+ * `device`: its P/O bits at threshold tau and the member lists of its rare
+ * alleles (count < tau, singletons included) stay in HBM; the per-locus id
+ * dictionaries stay on the host to place query ids in the same columns. */
+int cgh_db_create(int device, const uint64_t* calls, int n, int L, int tau, cgh_db** out, char* err, int errlen);
+int cgh_db_info(const cgh_db* db, int* n, int* L, int* tau, size_t* D);
+void cgh_db_free(cgh_db* db);
+
+/* Server over a database: a worker thread keeps up to `slots` query batches
+ * in flight (one HIP stream each).  Waiting requests are admitted in arrival
+ * order as slots and device memory free up; every admitted batch packs as
+ * many whole requests as fit in max_batch queries and in the HBM budget left
+ * by the batches in flight (hbm_budget bytes, 0 = what free memory allows),
+ * and runs as one query x database product, rare-pair pass and top-10
+ * selection.  kernel: CGH_KERNEL_*.  Calls that overlap on one server are
+ * safe (submit and wait may come from any thread). */
+int cgh_server_create(cgh_db* db, int slots, int max_batch, size_t hbm_budget, int kernel, cgh_server** out,
+                      char* err, int errlen);
+int cgh_server_slots(const cgh_server* sv);
+
+/* Submit q query profiles (q x L u64 ids, copied); returns a ticket >= 0, or
+ * -1 with err set. */
+long cgh_server_submit(cgh_server* sv, const uint64_t* calls, int q, char* err, int errlen);
+
+/* Block until the ticket's request is done; copy its q x n u16 distances
+ * (dist, may be NULL), q x 10 nearest database rows ascending by (distance,
+ * row) (top_d u16 / top_i u32, may be NULL) and its submit-to-done latency
+ * in seconds.  Each ticket is waited on once. */
+int cgh_server_wait(cgh_server* sv, long ticket, uint16_t* dist, uint16_t* top_d, uint32_t* top_i,
+                    double* latency_s, char* err, int errlen);
+void cgh_server_free(cgh_server* sv);
+
 /* Measured peaks of the units the kernels use, from register-only loops on
  * `device`: int8 MFMA (TOP/s), fp4 MFMA (TFLOP/s), VALU AND + popcount
  * (TOP/s, 2 ops per bit pair) and a streaming copy (GB/s, read + write).

@@ -125,3 +125,62 @@ def test_cli_device_and_separator_errors(cgh):
     assert r.returncode == 1 and r.stderr.startswith("error: GPU device 0 not available")
     r = subprocess.run([cli, "-i", "/nonexistent", "-d", "ab"], capture_output=True, text=True)
     assert r.returncode == 1 and "input separator must be a single character" in r.stderr
+
+
+def _bits(words, width):
+    """u32 rows -> 0/1 float32 matrix of the first `width` bits (LSB-first)."""
+    b = np.unpackbits(np.ascontiguousarray(words).view(np.uint8), axis=1, bitorder="little")
+    return b[:, :width].astype(np.float32)
+
+
+@pytest.mark.parametrize("tau", [1, 4, 32, 10**6])
+def test_query_codec_identity(cgh, workloads, tau):
+    """Queries encoded against a database must give, through
+    H = |P_q & P_j| - |O_q & O_j| - #ranges of q holding j, exactly the
+    oracle's brute-force distances of the two sets ranked together; the
+    database's own encoding must be cgh_encode's of its dense ranks."""
+    n_db, L, nq = 1500, 150, 97
+    db_calls, q_calls = workloads.serving_sets(n_db, L, nq, seed=31)
+    X_all = oracle.dense_ranks(np.vstack([db_calls, q_calls]))
+    want = oracle.brute_rows(X_all, n_db, n_db + nq)[:, :n_db].astype(np.int64)
+
+    codec = cgh.QueryCodec(db_calls, tau, want_encoding=True)
+    assert codec.tau == min(tau, n_db + 1)
+    own = cgh.Encoded(cgh.dense_ranks(db_calls), codec.tau, 256)
+    e, f = codec.enc, own.enc
+    assert (e.kw1, e.kw2, e.D) == (f.kw1, f.kw2, f.D) == (codec.kw1, codec.kw2, codec.D)
+    assert np.array_equal(own.O, np.ctypeslib.as_array(e.O, shape=(e.n_pad, e.kw1)))
+    assert np.array_equal(own.P, np.ctypeslib.as_array(e.P, shape=(e.n_pad, e.kw2)))
+
+    Oq, Pq, rs, re = codec.encode(q_calls)
+    assert rs[0] == 0 and rs[-1] == re.size and np.all(np.diff(rs.astype(np.int64)) >= 0)
+    mem = codec.members()
+    b, end = (re & 0xFFFFFFFF).astype(np.int64), (re >> np.uint64(32)).astype(np.int64)
+    assert np.all(b <= end) and np.all(end <= mem.size)
+    R = np.zeros((nq, n_db), dtype=np.int64)
+    for r in range(nq):
+        for k in range(int(rs[r]), int(rs[r + 1])):
+            R[r, mem[b[k]:end[k]]] += 1
+    Odb, Pdb = own.O[:n_db], own.P[:n_db]
+    H = (_bits(Pq, L) @ _bits(Pdb, L).T).astype(np.int64) - (
+        _bits(Oq, codec.D) @ _bits(Odb, codec.D).T).astype(np.int64) - R
+    assert np.array_equal(H, want)
+    # every list member carries the allele: lists partition the database's
+    # non-frequent calls (count < tau, singletons included)
+    assert mem.size == int((X_all[:n_db] > 0).sum()) - int(_bits(Odb, codec.D).sum())
+    codec.close()
+
+
+def test_query_codec_rejects_short_buffer(cgh, workloads):
+    db_calls, q_calls = workloads.serving_sets(200, 40, 5, seed=3)
+    codec = cgh.QueryCodec(db_calls, 8)
+    lib = cgh.load()
+    q = np.ascontiguousarray(q_calls)
+    O = np.zeros((5, codec.kw1), np.uint32)
+    P = np.zeros((5, codec.kw2), np.uint32)
+    rs = np.zeros(6, np.uint32)
+    re = np.zeros(5 * 40 - 1, np.uint64)
+    err = cgh._err()
+    assert lib.cgh_codec_encode(codec.handle, cgh._ptr(q), 5, cgh._ptr(O), cgh._ptr(P), cgh._ptr(rs), cgh._ptr(re),
+                                re.size, err, 512) == -1
+    assert b"rare entries" in err.value
