@@ -161,7 +161,11 @@ def run_serve(args, cgh, wl, torch):
     calls = wl.zipf_iid(n_db + int(sizes.sum()), Ls, 9, s=1.2, a_min=5, a_max=3000, miss=0.02)
     db_calls, q_calls = calls[:n_db], calls[n_db:]
     del calls
-    db = cgh.Database(db_calls, args.tau or 1024)
+    # tau by the upper-triangle cost model on the database itself: queries
+    # drawn from the same population meet its rare alleles in the same
+    # proportion (expected partners per query ~ sum c^2 / n)
+    tau = args.tau or cgh.choose_tau(cgh.dense_ranks(db_calls), 0, cgh.free_vram(0)) or 1024
+    db = cgh.Database(db_calls, tau)
     del db_calls
     log(f"serve: database {n_db}x{Ls} tau={db.tau} one-hot={db.D} resident in {time.perf_counter() - t:.1f}s; "
         f"{n_req} requests, {int(sizes.sum())} queries")
@@ -180,7 +184,7 @@ def run_serve(args, cgh, wl, torch):
                 d = t0 + at - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                tickets[k] = sv.submit(q_calls[bounds[k]:bounds[k + 1]])
+                tickets[k] = sv.submit(q_calls[bounds[k]:bounds[k + 1]], want_dist=False)
 
         th = threading.Thread(target=submitter)
         t0 = time.perf_counter()
@@ -189,7 +193,7 @@ def run_serve(args, cgh, wl, torch):
         for k in range(n_req):
             while tickets[k] is None:
                 time.sleep(0.0005)
-            lats.append(sv.wait(tickets[k], int(sizes[k]), want_dist=False)[3])
+            lats.append(sv.wait(tickets[k])[3])
         th.join()
         return time.perf_counter() - t0, np.array(lats)
 

@@ -38,6 +38,7 @@ EXPORTS = (
 )
 
 PHASE_PRODUCT, PHASE_RARE = 1, 2
+WANT_DIST = 1
 
 
 class CghError(RuntimeError):
@@ -105,7 +106,7 @@ def load():
         "cgh_db_free": (None, [P]),
         "cgh_server_create": (I, [P, I, I, S, I, C.POINTER(P), C.c_char_p, I]),
         "cgh_server_slots": (I, [P]),
-        "cgh_server_submit": (C.c_long, [P, P, I, C.c_char_p, I]),
+        "cgh_server_submit": (C.c_long, [P, P, I, I, C.c_char_p, I]),
         "cgh_server_wait": (I, [P, C.c_long, P, P, P, C.POINTER(C.c_double), C.c_char_p, I]),
         "cgh_server_free": (None, [P]),
         "cgh_dataset_tile": (I, [P, I]),
@@ -412,6 +413,7 @@ class Server:
         _check(load().cgh_server_create(db.handle, slots, max_batch, hbm_budget, kernel, C.byref(self.handle), err,
                                         512), err)
         self.slots = load().cgh_server_slots(self.handle)
+        self._tickets = {}
 
     def close(self):
         if _lib is not None and self.handle:
@@ -420,16 +422,20 @@ class Server:
 
     __del__ = close
 
-    def submit(self, calls: np.ndarray) -> int:
+    def submit(self, calls: np.ndarray, want_dist: bool = True) -> int:
         calls = np.ascontiguousarray(calls, dtype=np.uint64)
         err = _err()
-        t = load().cgh_server_submit(self.handle, _ptr(calls), calls.shape[0], err, 512)
+        t = load().cgh_server_submit(self.handle, _ptr(calls), calls.shape[0], WANT_DIST if want_dist else 0, err,
+                                     512)
         if t < 0:
             raise CghError(err.value.decode(errors="replace"))
+        self._tickets[t] = (calls.shape[0], want_dist)
         return t
 
-    def wait(self, ticket: int, q: int, want_dist: bool = True):
-        """-> (dist q x n u16 or None, top_d q x 10, top_i q x 10, latency s)"""
+    def wait(self, ticket: int):
+        """-> (dist q x n u16, or None when submitted without it; top_d q x
+        10; top_i q x 10; latency s)"""
+        q, want_dist = self._tickets.pop(ticket)
         dist = np.zeros((q, self.db.n), dtype=np.uint16) if want_dist else None
         top_d = np.zeros((q, 10), dtype=np.uint16)
         top_i = np.zeros((q, 10), dtype=np.uint32)

@@ -1246,6 +1246,7 @@ struct cgh_server {
     struct Request {
         long id;
         int q;
+        bool want_dist;
         std::vector<uint64_t> calls;
         double t_submit = 0, t_done = 0;
         std::vector<uint16_t> dist, top_d;
@@ -1339,7 +1340,13 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     }
     hipLaunchKernelGGL(top_k_rows, dim3((unsigned)q), dim3(256), 0, sl.st, sl.ddist, db->n, sl.dtopd, sl.dtopi);
     if ((e = hipGetLastError())) return e;
-    if ((e = hipMemcpyAsync(sl.hdist, sl.ddist, (size_t)q * db->n * 2, hipMemcpyDeviceToHost, sl.st))) return e;
+    row = 0;  // distances only of the requests that asked for them
+    for (auto* r : sl.reqs) {
+        const size_t off = row * (size_t)db->n, cnt = (size_t)r->q * db->n;
+        if (r->want_dist && (e = hipMemcpyAsync(sl.hdist + off, sl.ddist + off, cnt * 2, hipMemcpyDeviceToHost, sl.st)))
+            return e;
+        row += (size_t)r->q;
+    }
     if ((e = hipMemcpyAsync(sl.htopd, sl.dtopd, (size_t)q * TOPK * 2, hipMemcpyDeviceToHost, sl.st))) return e;
     if ((e = hipMemcpyAsync(sl.htopi, sl.dtopi, (size_t)q * TOPK * 4, hipMemcpyDeviceToHost, sl.st))) return e;
     return hipEventRecord(sl.ev, sl.st);
@@ -1347,9 +1354,13 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
 
 // Worker: retire finished batches, then give every free slot the longest
 // run of waiting requests (arrival order) that fits max_batch queries and the
-// HBM budget left by the batches in flight.
+// HBM budget left by the batches in flight.  Only the worker touches slots
+// and the requests it has taken from the queue, so result copies, query
+// encoding and launches run with the lock released; a waiter sees a request
+// only once `done` is set under the lock.
 void serve_loop(cgh_server* sv) {
     (void)hipSetDevice(sv->db->device);
+    const size_t n = (size_t)sv->db->n;
     std::unique_lock<std::mutex> lk(sv->mu);
     while (!sv->stop) {
         bool progressed = false;
@@ -1357,17 +1368,21 @@ void serve_loop(cgh_server* sv) {
             if (!sl.busy) continue;
             const hipError_t qe = hipEventQuery(sl.ev);
             if (qe == hipErrorNotReady) continue;
+            lk.unlock();
             size_t row = 0;
-            const double t = now_s();
             for (auto* r : sl.reqs) {
                 if (qe != hipSuccess) {
                     r->err = hipGetErrorString(qe);
                 } else {
-                    r->dist.assign(sl.hdist + row * sv->db->n, sl.hdist + (row + r->q) * sv->db->n);
+                    if (r->want_dist) r->dist.assign(sl.hdist + row * n, sl.hdist + (row + r->q) * n);
                     r->top_d.assign(sl.htopd + row * TOPK, sl.htopd + (row + r->q) * TOPK);
                     r->top_i.assign(sl.htopi + row * TOPK, sl.htopi + (row + r->q) * TOPK);
                 }
                 row += (size_t)r->q;
+            }
+            const double t = now_s();
+            lk.lock();
+            for (auto* r : sl.reqs) {
                 r->t_done = t;
                 r->done = true;
             }
@@ -1393,7 +1408,9 @@ void serve_loop(cgh_server* sv) {
             sl.bytes = batch_bytes(sv->db, q);
             sv->inflight += sl.bytes;
             sl.busy = true;
+            lk.unlock();
             const hipError_t e = launch_batch(sv, sl);
+            lk.lock();
             if (e != hipSuccess) {
                 for (auto* r : sl.reqs) {
                     r->err = hipGetErrorString(e);
@@ -1508,7 +1525,7 @@ fail:
 
 extern "C" int cgh_server_slots(const cgh_server* sv) { return sv->nslots; }
 
-extern "C" long cgh_server_submit(cgh_server* sv, const uint64_t* calls, int q, char* err, int errlen) {
+extern "C" long cgh_server_submit(cgh_server* sv, const uint64_t* calls, int q, int flags, char* err, int errlen) {
     if (q < 1 || q > sv->max_batch) {
         set_err(err, errlen, "request of %d queries (1..%d accepted)", q, sv->max_batch);
         return -1;
@@ -1520,6 +1537,7 @@ extern "C" long cgh_server_submit(cgh_server* sv, const uint64_t* calls, int q, 
     }
     cgh_server::Request* r = new cgh_server::Request();
     r->q = q;
+    r->want_dist = (flags & CGH_WANT_DIST) != 0;
     r->calls.assign(calls, calls + (size_t)q * sv->db->L);
     {
         std::lock_guard<std::mutex> lk(sv->mu);
@@ -1541,6 +1559,10 @@ extern "C" int cgh_server_wait(cgh_server* sv, long ticket, uint16_t* dist, uint
         return 1;
     }
     cgh_server::Request* r = it->second;
+    if (dist && !r->want_dist) {
+        set_err(err, errlen, "ticket %ld was submitted without CGH_WANT_DIST", ticket);
+        return 1;
+    }
     sv->cv.wait(lk, [&] { return r->done; });
     sv->reqs.erase(it);
     lk.unlock();

@@ -250,13 +250,16 @@ int cgh_server_create(cgh_db* db, int slots, int max_batch, size_t hbm_budget, i
 int cgh_server_slots(const cgh_server* sv);
 
 /* Submit q query profiles (q x L u64 ids, copied); returns a ticket >= 0, or
- * -1 with err set. */
-long cgh_server_submit(cgh_server* sv, const uint64_t* calls, int q, char* err, int errlen);
+ * -1 with err set.  flags: CGH_WANT_DIST to get the q x n distances back as
+ * well as the nearest rows (without it they stay on the device). */
+#define CGH_WANT_DIST 1
+long cgh_server_submit(cgh_server* sv, const uint64_t* calls, int q, int flags, char* err, int errlen);
 
 /* Block until the ticket's request is done; copy its q x n u16 distances
- * (dist, may be NULL), q x 10 nearest database rows ascending by (distance,
- * row) (top_d u16 / top_i u32, may be NULL) and its submit-to-done latency
- * in seconds.  Each ticket is waited on once. */
+ * (dist, may be NULL; non-NULL needs CGH_WANT_DIST at submit), q x 10
+ * nearest database rows ascending by (distance, row) (top_d u16 / top_i u32,
+ * may be NULL; rows past n read 0xffff / 0xffffffff) and its submit-to-done
+ * latency in seconds.  Each ticket is waited on once. */
 int cgh_server_wait(cgh_server* sv, long ticket, uint16_t* dist, uint16_t* top_d, uint32_t* top_i,
                     double* latency_s, char* err, int errlen);
 void cgh_server_free(cgh_server* sv);

@@ -40,17 +40,18 @@ def test_server_alone_and_mixed(cgh, gpu, workloads, tau, slots, kernel):
     bounds = np.cumsum([0] + SIZES)
     # mixed: everything submitted at once, batches overlap on the device
     print(f"server tau={tau} slots={sv.slots} kernel={kernel} mixed", flush=True)
-    tickets = [sv.submit(q_calls[bounds[k]:bounds[k + 1]]) for k in range(len(SIZES))]
-    mixed = [sv.wait(t, SIZES[k]) for k, t in enumerate(tickets)]
+    tickets = [sv.submit(q_calls[bounds[k]:bounds[k + 1]], want_dist=k % 2 == 0) for k in range(len(SIZES))]
+    mixed = [sv.wait(t) for t in tickets]
     # alone: one request in flight at a time
     print(f"server tau={tau} alone", flush=True)
-    alone = [sv.wait(sv.submit(q_calls[bounds[k]:bounds[k + 1]]), SIZES[k]) for k in range(len(SIZES))]
+    alone = [sv.wait(sv.submit(q_calls[bounds[k]:bounds[k + 1]])) for k in range(len(SIZES))]
     for k in range(len(SIZES)):
         want = ref[bounds[k]:bounds[k + 1]]
         td, ti = expected_top(want)
         for res in (mixed[k], alone[k]):
             dist, top_d, top_i, lat = res
-            assert np.array_equal(dist, want), k
+            assert (dist is None) == (res is mixed[k] and k % 2 == 1), k
+            assert dist is None or np.array_equal(dist, want), k
             assert np.array_equal(top_d, td) and np.array_equal(top_i, ti), k
             assert lat > 0
     sv.close()
@@ -72,7 +73,23 @@ def test_server_admission_by_memory(cgh, gpu, workloads):
     print("server admission", flush=True)
     tickets = {k: sv.submit(q_calls[bounds[k]:bounds[k + 1]]) for k in small}
     for k, t in tickets.items():
-        dist, top_d, top_i, _ = sv.wait(t, SIZES[k])
+        dist, top_d, top_i, _ = sv.wait(t)
         assert np.array_equal(dist, ref[bounds[k]:bounds[k + 1]]), k
+    sv.close()
+    db.close()
+
+
+def test_server_database_smaller_than_top_k(cgh, gpu, workloads):
+    """7 database rows: the 10-entry lists end with (0xffff, 0xffffffff)."""
+    db_calls, q_calls = workloads.serving_sets(7, 60, 3, seed=9)
+    X_all = oracle.dense_ranks(np.vstack([db_calls, q_calls]))
+    ref = oracle.brute_rows(X_all, 7, 10)[:, :7]
+    db = cgh.Database(db_calls, 2)
+    sv = cgh.Server(db, slots=1, max_batch=4)
+    dist, top_d, top_i, _ = sv.wait(sv.submit(q_calls))
+    assert np.array_equal(dist, ref)
+    td, ti = expected_top(ref)
+    assert np.array_equal(top_d[:, :7], td[:, :7]) and np.array_equal(top_i[:, :7], ti[:, :7])
+    assert np.all(top_d[:, 7:] == 0xFFFF) and np.all(top_i[:, 7:] == 0xFFFFFFFF)
     sv.close()
     db.close()
