@@ -1139,37 +1139,37 @@ namespace {
 
 constexpr int TOPK = 10;
 
-// k nearest database rows of each query row of dist (rows x nB u16):
-// ascending (distance, row index).  One workgroup per query: every thread
-// keeps a sorted list of its best TOPK, the lists meet in LDS and wavefront 0
-// draws the TOPK overall minima one by one.
-__global__ void __launch_bounds__(256)
-top_k_rows(const uint16_t* __restrict__ dist, int nB, uint16_t* __restrict__ top_d, uint32_t* __restrict__ top_i) {
-    __shared__ uint64_t cand[256 * TOPK];
-    const int q = blockIdx.x, tid = threadIdx.x;
-    const uint16_t* row = dist + (size_t)q * nB;
-    uint64_t best[TOPK];
-#pragma unroll
-    for (int k = 0; k < TOPK; k++) best[k] = ~0ull;
-    for (int j = tid; j < nB; j += 256) {
-        const uint64_t key = ((uint64_t)row[j] << 32) | (uint32_t)j;
-        if (key < best[TOPK - 1]) {
-            int k = TOPK - 1;
-            while (k > 0 && best[k - 1] > key) {
-                best[k] = best[k - 1];
-                k--;
-            }
-            best[k] = key;
+// k nearest database rows of each query row of dist (rows x nB u16),
+// ascending (distance, row index), as 64-bit keys dist << 32 | row.  Two
+// levels so that every query row is spread over many workgroups: a part
+// kernel reduces each TKC-column chunk of a row to its TOPK smallest keys,
+// a final kernel reduces a row's chunk lists.  Inside a 256-thread
+// workgroup every thread keeps a sorted list of its best TOPK, the lists
+// meet in LDS and wavefront 0 draws the TOPK overall minima one by one.
+constexpr int TKC = 8192;  // columns per part workgroup (32 per thread)
+
+__device__ __forceinline__ void keep_best(uint64_t (&best)[TOPK], uint64_t key) {
+    if (key < best[TOPK - 1]) {
+        int k = TOPK - 1;
+        while (k > 0 && best[k - 1] > key) {
+            best[k] = best[k - 1];
+            k--;
         }
+        best[k] = key;
     }
+}
+
+// The TOPK smallest of the 256 threads' sorted lists -> out[0..TOPK) (thread 0
+// writes; keys are unique, so a head advances only in the lane that held the
+// minimum).
+__device__ __forceinline__ void block_best(const uint64_t (&best)[TOPK], uint64_t* cand, uint64_t* out) {
+    const int tid = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < TOPK; k++) cand[tid * TOPK + k] = best[k];
     __syncthreads();
     if (tid >= 64) return;
-    // lane l owns candidates [l*40, l*40+40) (sorted runs of 10): merge by
-    // repeated wavefront minimum
-    int head[4] = {0, 0, 0, 0};
-    for (int out = 0; out < TOPK; out++) {
+    int head[4] = {0, 0, 0, 0};  // lane l merges the lists of threads 4l .. 4l+3
+    for (int o = 0; o < TOPK; o++) {
         uint64_t mine = ~0ull;
         int which = -1;
         for (int r = 0; r < 4; r++) {
@@ -1181,14 +1181,46 @@ top_k_rows(const uint16_t* __restrict__ dist, int nB, uint16_t* __restrict__ top
         }
         uint64_t m = mine;
         for (int off = 32; off > 0; off >>= 1) {
-            const uint64_t o = __shfl_xor(m, off, 64);
-            m = o < m ? o : m;
+            const uint64_t x = __shfl_xor(m, off, 64);
+            m = x < m ? x : m;
         }
-        if (which >= 0 && mine == m) head[which]++;  // keys are unique (row index)
-        if (tid == 0) {
-            top_d[(size_t)q * TOPK + out] = m == ~0ull ? 0xffff : (uint16_t)(m >> 32);
-            top_i[(size_t)q * TOPK + out] = m == ~0ull ? 0xffffffffu : (uint32_t)m;
-        }
+        if (which >= 0 && mine == m) head[which]++;
+        if (tid == 0) out[o] = m;
+    }
+}
+
+// Workgroup b: row b / chunks, columns [TKC * (b % chunks), +TKC).
+__global__ void __launch_bounds__(256)
+top_k_part(const uint16_t* __restrict__ dist, int nB, int chunks, uint64_t* __restrict__ part) {
+    __shared__ uint64_t cand[256 * TOPK];
+    const int row = blockIdx.x / chunks, ch = blockIdx.x % chunks;
+    const uint16_t* d = dist + (size_t)row * nB;
+    const int c1 = min(nB, (ch + 1) * TKC);
+    uint64_t best[TOPK];
+#pragma unroll
+    for (int k = 0; k < TOPK; k++) best[k] = ~0ull;
+    for (int j = ch * TKC + (int)threadIdx.x; j < c1; j += 256) keep_best(best, ((uint64_t)d[j] << 32) | (uint32_t)j);
+    block_best(best, cand, part + (size_t)blockIdx.x * TOPK);
+}
+
+// Workgroup q: the chunks * TOPK keys of row q -> its TOPK nearest rows.
+__global__ void __launch_bounds__(256)
+top_k_final(const uint64_t* __restrict__ part, int chunks, uint16_t* __restrict__ top_d,
+            uint32_t* __restrict__ top_i) {
+    __shared__ uint64_t cand[256 * TOPK];
+    __shared__ uint64_t res[TOPK];
+    const int q = blockIdx.x;
+    const uint64_t* p = part + (size_t)q * chunks * TOPK;
+    uint64_t best[TOPK];
+#pragma unroll
+    for (int k = 0; k < TOPK; k++) best[k] = ~0ull;
+    for (int j = threadIdx.x; j < chunks * TOPK; j += 256) keep_best(best, p[j]);
+    block_best(best, cand, res);
+    __syncthreads();
+    if (threadIdx.x < TOPK) {
+        const uint64_t m = res[threadIdx.x];
+        top_d[(size_t)q * TOPK + threadIdx.x] = m == ~0ull ? 0xffff : (uint16_t)(m >> 32);
+        top_i[(size_t)q * TOPK + threadIdx.x] = m == ~0ull ? 0xffffffffu : (uint32_t)m;
     }
 }
 
@@ -1264,6 +1296,7 @@ struct cgh_server {
         uint32_t *dO = NULL, *dP = NULL, *drs = NULL;
         uint64_t* dre = NULL;
         uint16_t *ddist = NULL, *dtopd = NULL;
+        uint64_t* dpart = NULL;  // per-chunk nearest-row keys
         uint32_t* dtopi = NULL;
         uint32_t *hO = NULL, *hP = NULL, *hrs = NULL;
         uint64_t* hre = NULL;
@@ -1293,7 +1326,8 @@ double now_s() {
 
 size_t batch_bytes(const cgh_db* db, int q) {
     return (size_t)q * (size_t)db->n * 2 + (size_t)up((size_t)q, 256) * (db->kw1 + db->kw2) * 4 +
-           (size_t)q * (size_t)db->L * 8 + (size_t)q * TOPK * 6;
+           (size_t)q * (size_t)db->L * 8 + (size_t)q * TOPK * 6 +
+           (size_t)q * (size_t)((db->n + TKC - 1) / TKC) * TOPK * 8;
 }
 
 // Enqueue one batch of queries on its slot: upload, product, rare pairs,
@@ -1338,7 +1372,10 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
         const RareJob jb = {sl.ddist, q, 0, q, chunks, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n};
         hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)q * chunks)), dim3(RNT), 0, sl.st, jb);
     }
-    hipLaunchKernelGGL(top_k_rows, dim3((unsigned)q), dim3(256), 0, sl.st, sl.ddist, db->n, sl.dtopd, sl.dtopi);
+    const int tchunks = (db->n + TKC - 1) / TKC;
+    hipLaunchKernelGGL(top_k_part, dim3((unsigned)((size_t)q * tchunks)), dim3(256), 0, sl.st, sl.ddist, db->n,
+                       tchunks, sl.dpart);
+    hipLaunchKernelGGL(top_k_final, dim3((unsigned)q), dim3(256), 0, sl.st, sl.dpart, tchunks, sl.dtopd, sl.dtopi);
     if ((e = hipGetLastError())) return e;
     row = 0;  // distances only of the requests that asked for them
     for (auto* r : sl.reqs) {
@@ -1449,7 +1486,7 @@ extern "C" void cgh_server_free(cgh_server* sv) {
     (void)hipSetDevice(sv->db->device);
     for (auto& sl : sv->slots) {
         if (sl.st) (void)hipStreamSynchronize(sl.st);
-        void* dev[] = {sl.dO, sl.dP, sl.drs, sl.dre, sl.ddist, sl.dtopd, sl.dtopi};
+        void* dev[] = {sl.dO, sl.dP, sl.drs, sl.dre, sl.ddist, sl.dtopd, sl.dtopi, sl.dpart};
         for (void* p : dev)
             if (p) (void)hipFree(p);
         void* host[] = {sl.hO, sl.hP, sl.hrs, sl.hre, sl.hdist, sl.htopd, sl.htopi};
@@ -1507,6 +1544,7 @@ extern "C" int cgh_server_create(cgh_db* db, int slots, int max_batch, size_t hb
         HK(hipMalloc(&sl.ddist, up((size_t)max_batch * db->n, 8) * 2));
         HK(hipMalloc(&sl.dtopd, (size_t)max_batch * TOPK * 2));
         HK(hipMalloc(&sl.dtopi, (size_t)max_batch * TOPK * 4));
+        HK(hipMalloc(&sl.dpart, (size_t)max_batch * ((db->n + TKC - 1) / TKC) * TOPK * 8));
         HK(hipHostMalloc(&sl.hO, qp * db->kw1 * 4, hipHostMallocDefault));
         HK(hipHostMalloc(&sl.hP, qp * db->kw2 * 4, hipHostMallocDefault));
         HK(hipHostMalloc(&sl.hrs, ((size_t)max_batch + 1) * 4, hipHostMallocDefault));
