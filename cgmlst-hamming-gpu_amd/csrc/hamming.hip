@@ -2,19 +2,27 @@
 //
 //   H = P P^T - O O^T - R        (exact, integer arithmetic throughout)
 //
-// The binary products run on int8 MFMA (v_mfma_i32_32x32x32_i8) over 0/1
-// operands expanded on the fly from bit-packed rows: every workgroup stages a
-// KC-word slab of its A rows and B rows in LDS as raw 32-bit words (k-major,
-// so a wavefront reads 32 consecutive rows of one word without bank
-// conflicts), and each wavefront expands a word into the 16 int8 bytes of its
-// MFMA operand with shifts and masks.  The same bit -> (lane half, byte) map
-// is used for A and B, so every bit of the word is counted exactly once.
-// One accumulator carries the whole identity: the O slab is accumulated
-// first, the accumulator is negated, then the P slab is added, so the
-// epilogue holds P P^T - O O^T and writes u16 directly (no int32 stripe, no
-// separate pack pass).  Rare equal pairs are subtracted afterwards with
-// 32-bit atomics on the packed u16 cells (a cell never drops below its final
-// value, so the half-word subtraction never borrows into its neighbour).
+// Binary products.  O (frequent-allele one-hot bits) and P (presence bits)
+// live in HBM as bit planes: plane g holds 128 bits (words 4g..4g+3) of every
+// row, 16 bytes per row, rows contiguous, so the 16-byte pieces of 64
+// consecutive rows are one contiguous 1 KiB block.  A workgroup computes a
+// 256 x 256 (or 128 x 128) output tile; each step stages a KC-word slab of its
+// A rows and B rows in an LDS ring straight from global memory
+// (global_load_lds_dwordx4: one 1 KiB block per wavefront load, no register
+// round trip), NST-1 slabs ahead of the one being multiplied.  Every
+// wavefront expands the words of its rows into MFMA operands in registers:
+// fp4 (e2m1 0 / +1 / -1, v_mfma_scale_f32_32x32x64_f8f6f4, 64 bits of k per
+// instruction, fp32 accumulation of integers <= L: exact) or int8 (0/1,
+// v_mfma_i32_32x32x32_i8).  The same bit -> operand position map is used for
+// A and B, so every bit is counted exactly once.  One accumulator carries
+// O O^T - P P^T (the P slabs multiply a negated B operand) and the epilogue
+// stores its negation as u16, zeroing cells with j <= i: no int32 stripe and
+// no pack pass.
+//
+// Rare equal pairs (alleles with count < tau) are subtracted afterwards, row
+// by row: a row's output cells are staged in LDS, every partner j listed for
+// the row is subtracted there with an LDS atomic, and the row is written back
+// once (HBM sees each cell read and written once).
 //
 // Reference structure this replaces: tensorcore/rs/cuda/tch.cu:22-66 (CUTLASS
 // 1-bit GEMM, two launches with beta = -1), 36-43 (sub_rare), 45-50
@@ -46,7 +54,7 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-constexpr int KC = 8;  // 32-bit words per row staged per pipeline step (256 bits)
+constexpr int KC = 16;  // 32-bit words per row staged per pipeline step (512 bits)
 
 // One launch covers `ns` consecutive stripes of S rows starting at row r0.
 // Stripe s (rows r0 + s*S ..) is stored at out + stripe_off(n - r0, S, s) as
@@ -56,8 +64,8 @@ constexpr int KC = 8;  // 32-bit words per row staged per pipeline step (256 bit
 // is a multiple of BM, so a full stripe has q = S/BM row tiles and nbc - s*q
 // column tiles, and the tile count before stripe s has a closed form.
 struct TileJob {
-    const uint32_t* O;  // n_pad x kw1 words (frequent-allele one-hot bits)
-    const uint32_t* P;  // n_pad x kw2 words (presence bits)
+    const uint32_t* O;  // kw1 / 4 planes of ldA rows (frequent-allele one-hot bits)
+    const uint32_t* P;  // kw2 / 4 planes of ldA rows (presence bits)
     int kw1, kw2;       // words per row, multiples of KC
     int n;              // valid rows
     int r0;             // first row of the layout
@@ -69,6 +77,7 @@ struct TileJob {
     const uint32_t* OB;
     const uint32_t* PB;
     int nB, rect;
+    int ldA, ldB;       // rows per plane of (O, P) and of (OB, PB)
 };
 
 struct TileArgs {
@@ -153,7 +162,7 @@ __device__ __forceinline__ Acc mfma_fp4(v8i a, v8i b, Acc c) {
 // Square BM x BM output tile per workgroup, WM x WN per wavefront; FP4
 // selects the fp4 MFMA (64 bits of k per instruction) over the int8 one (32).
 constexpr int KG = KC / 4;   // 4-word groups per slab
-constexpr int NST = 4;       // LDS ring stages (slabs in flight)
+constexpr int NST = 3;       // LDS ring stages (slabs in flight)
 
 // ring[stage][side A/B][group][row] of 4 consecutive words: one
 // wavefront-wide direct-to-LDS load fills 64 consecutive rows of a group
@@ -202,9 +211,9 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     // stored.
     auto issue_slab = [&](int c, int st) {
         const uint32_t *baseA, *baseB;
-        int kw, k0;
-        if (c < nc1) { baseA = a.O; baseB = a.OB; kw = a.kw1; k0 = c * KC; }
-        else { baseA = a.P; baseB = a.PB; kw = a.kw2; k0 = (c - nc1) * KC; }
+        int k0;
+        if (c < nc1) { baseA = a.O; baseB = a.OB; k0 = c * KC; }
+        else { baseA = a.P; baseB = a.PB; k0 = (c - nc1) * KC; }
 #pragma unroll
         for (int u = 0; u < LPW; u++) {
             const int v = wave * LPW + u;
@@ -212,7 +221,8 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
             const int rem = v % (KG * (BM / 64));
             const int grp = rem / (BM / 64), rb = rem % (BM / 64);
             const int grow = side ? min(growB + rb * 64 + lane, a.nB - 1) : min(growA + rb * 64 + lane, a.n - 1);
-            const uint32_t* src = (side ? baseB : baseA) + (size_t)grow * kw + k0 + grp * 4;
+            const uint32_t* src =
+                (side ? baseB : baseA) + ((size_t)(k0 / 4 + grp) * (size_t)(side ? a.ldB : a.ldA) + grow) * 4;
             // LDS base of this wavefront's 64 rows goes to M0.  Written as an
             // instruction rather than the builtin so that the compiler does
             // not wait for every LDS-DMA transfer before each LDS read (it
@@ -249,6 +259,7 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     // fp4: each lane half g reads words 2g, 2g+1 of a 4-word group (one
     // 8-byte read), one word per 64-deep MFMA step.
     auto compute_slab_fp4 = [&](int buf, auto neg) {
+        __builtin_amdgcn_iglp_opt(1);  // interleave the LDS reads and expansions with the MFMAs
         uint2 wa[TM], wb[TN];
 #pragma unroll
         for (int i = 0; i < TM; i++) wa[i] = reinterpret_cast<const uint2*>(&ring[buf][0][0][wm0 + i * 32 + l32])[g];
@@ -366,6 +377,10 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
         }
 }
 
+// Tile shapes used: 256 x 256 by 8 wavefronts of 128 x 64 (two per SIMD, so
+// one wavefront's LDS and barrier waits are covered by the other's MFMAs) and
+// 128 x 128 by 4 wavefronts of 64 x 64.
+//
 // One launch over the closed-form tile numbering of TileArgs.  Workgroups
 // are dealt round-robin to the 8 XCDs; each XCD gets a contiguous run of
 // tiles so concurrently resident tiles share panels in that XCD's L2.
@@ -410,6 +425,8 @@ __device__ __forceinline__ void scalarize(TileJob& j) {
     j.S = __builtin_amdgcn_readfirstlane(j.S);
     j.nB = __builtin_amdgcn_readfirstlane(j.nB);
     j.rect = __builtin_amdgcn_readfirstlane(j.rect);
+    j.ldA = __builtin_amdgcn_readfirstlane(j.ldA);
+    j.ldB = __builtin_amdgcn_readfirstlane(j.ldB);
 }
 
 // Persistent form for many datasets: each workgroup pulls tiles (job,
@@ -665,11 +682,57 @@ int pick_tile(int n) { return n >= 8192 ? 256 : 128; }
 
 // Stripes r0, r0+S, ... up to row n in one launch when S is a multiple of the
 // tile (else one launch per stripe), output in the stripe layout at out.
-hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int n, int r0,
-                        int S, int ns, uint16_t* out, hipStream_t st) {
+// Row-major operand (rows x kw words) -> planes: plane g holds words
+// 4g..4g+3 of every row (16 bytes per row, ld rows per plane).  Planes past
+// kw / 4 and rows past `rows` read zero.  One thread per 16-byte piece, in
+// plane order, so the stores are contiguous.
+__global__ void __launch_bounds__(256)
+to_planes(const uint32_t* __restrict__ src, int rows, int kw, uint4* __restrict__ dst, int ld, int planes) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)planes * (size_t)ld) return;
+    const int g = (int)(idx / (size_t)ld), r = (int)(idx % (size_t)ld);
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (r < rows && 4 * g + 3 < kw) {
+        const uint32_t* p = src + (size_t)r * kw + 4 * g;
+        v = make_uint4(p[0], p[1], p[2], p[3]);
+    }
+    dst[idx] = v;
+}
+
+hipError_t launch_planes(const uint32_t* src, int rows, int kw, uint32_t* dst, int ld, int kwd, hipStream_t st) {
+    const size_t pieces = (size_t)(kwd / 4) * (size_t)ld;
+    hipLaunchKernelGGL(to_planes, dim3((unsigned)((pieces + 255) / 256)), dim3(256), 0, st, src, rows, kw,
+                       reinterpret_cast<uint4*>(dst), ld, kwd / 4);
+    return hipGetLastError();
+}
+
+// Host row-major operand (ld x kw words) -> a new device buffer in planes of
+// kwd words (kwd >= kw, a multiple of KC).
+hipError_t upload_planes(const void* host, int ld, int kw, int kwd, uint32_t** out) {
+    uint32_t* tmp = NULL;
+    hipError_t e;
+    *out = NULL;
+    if ((e = hipMalloc(&tmp, (size_t)ld * kw * 4))) return e;
+    if ((e = hipMalloc(out, (size_t)ld * kwd * 4))) goto done;
+    if ((e = hipMemcpy(tmp, host, (size_t)ld * kw * 4, hipMemcpyHostToDevice))) goto done;
+    if ((e = launch_planes(tmp, ld, kw, *out, ld, kwd, NULL))) goto done;
+    e = hipDeviceSynchronize();
+done:
+    (void)hipFree(tmp);
+    if (e && *out) {
+        (void)hipFree(*out);
+        *out = NULL;
+    }
+    return e;
+}
+
+int kc_words(size_t kw) { return (int)up(kw, (size_t)KC); }
+
+hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int ld, int n,
+                        int r0, int S, int ns, uint16_t* out, hipStream_t st) {
     if (ns > 1 && S % tile) {
         for (int s = 0; s < ns; s++) {
-            const hipError_t e = launch_tiles(kernel, tile, O, kw1, P, kw2, n, r0 + s * S, S, 1,
+            const hipError_t e = launch_tiles(kernel, tile, O, kw1, P, kw2, ld, n, r0 + s * S, S, 1,
                                               out + stripe_off((size_t)(n - r0), (size_t)S, (size_t)s), st);
             if (e != hipSuccess) return e;
         }
@@ -679,6 +742,7 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
     a.job.O = O; a.job.P = P; a.job.kw1 = kw1; a.job.kw2 = kw2; a.job.n = n; a.job.r0 = r0; a.job.S = S;
     a.job.out = out;
     a.job.OB = O; a.job.PB = P; a.job.nB = n; a.job.rect = 0;
+    a.job.ldA = ld; a.job.ldB = ld;
     a.ns = ns;
     a.nbc = (n - r0 + tile - 1) / tile;
     a.q = S / tile;
@@ -689,9 +753,9 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
     const dim3 grid((unsigned)(a.chunk * 8));
     const bool fp4 = kernel == CGH_KERNEL_MFMA_FP4;
     if (tile == 256 && fp4)
-        hipLaunchKernelGGL((mfma_tile<256, 128, 128, true>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((mfma_tile<256, 128, 64, true>), grid, dim3(512), 0, st, a);
     else if (tile == 256)
-        hipLaunchKernelGGL((mfma_tile<256, 128, 128, false>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((mfma_tile<256, 128, 64, false>), grid, dim3(512), 0, st, a);
     else if (fp4)
         hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, st, a);
     else
@@ -743,10 +807,9 @@ extern "C" int tch_run(int device, int n, size_t n_pad, const uint8_t* O, size_t
         return 1;
     }
     HK(hipSetDevice(device));
-    HK(hipMalloc(&dO, n_pad * K1 / 8));
-    HK(hipMalloc(&dP, n_pad * K2 / 8));
-    HK(hipMemcpy(dO, O, n_pad * K1 / 8, hipMemcpyHostToDevice));
-    HK(hipMemcpy(dP, P, n_pad * K2 / 8, hipMemcpyHostToDevice));
+    // the caller's row-major bit matrices -> device planes
+    HK(upload_planes(O, (int)n_pad, (int)(K1 / 32), kc_words(K1 / 32), &dO));
+    HK(upload_planes(P, (int)n_pad, (int)(K2 / 32), kc_words(K2 / 32), &dP));
     HK(hipMemGetInfo(&freeb, &totb));
     {
         const char* env = getenv("CGH_FREE_BYTES");
@@ -774,8 +837,8 @@ extern "C" int tch_run(int device, int n, size_t n_pad, const uint8_t* O, size_t
         if (r0 < n) {
             const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
             const size_t w = (size_t)(n - r0);
-            HK(launch_tiles(resolve_kernel(CGH_KERNEL_AUTO), tile, dO, (int)(K1 / 32), dP, (int)(K2 / 32), n, r0, S, 1,
-                            dU[b], st[b]));
+            HK(launch_tiles(resolve_kernel(CGH_KERNEL_AUTO), tile, dO, kc_words(K1 / 32), dP, kc_words(K2 / 32),
+                            (int)n_pad, n, r0, S, 1, dU[b], st[b]));
             if (rare_gpu && n_rare) {
                 hipLaunchKernelGGL(sub_rare_csr, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st[b], dU[b],
                                    (int)w, r0, rows, dStart, dJ);
@@ -826,8 +889,8 @@ fail:
 struct cgh_dataset {
     int device;
     int n, L;
-    size_t n_pad, kw1, kw2;
-    uint32_t *dO, *dP;
+    size_t n_pad, kw1, kw2;   // kw1, kw2: words per row on the device (multiples of KC)
+    uint32_t *dO, *dP;        // planes of n_pad rows
     size_t n_groups, n_members;
     uint64_t n_rare;
     size_t n_rentries;
@@ -857,22 +920,20 @@ extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_datas
     ds->n = enc->n;
     ds->L = enc->L;
     ds->n_pad = enc->n_pad;
-    ds->kw1 = enc->kw1;
-    ds->kw2 = enc->kw2;
+    ds->kw1 = (size_t)kc_words(enc->kw1);
+    ds->kw2 = (size_t)kc_words(enc->kw2);
     ds->n_groups = enc->n_groups;
     ds->n_members = enc->n_members;
     ds->n_rare = enc->n_rare;
     ds->n_rentries = enc->n_rentries;
-    if (enc->kw1 % KC || enc->kw2 % KC || enc->kw1 == 0 || enc->n_pad % 256) {
+    if (enc->kw1 % 4 || enc->kw2 % 4 || enc->kw1 == 0 || enc->n_pad % 256) {
         set_err(err, errlen, "cgh_dataset_upload: rows must be padded to 256 bits and 256 rows");
         free(ds);
         return 1;
     }
     HK(hipSetDevice(device));
-    HK(hipMalloc(&ds->dO, enc->n_pad * enc->kw1 * 4));
-    HK(hipMalloc(&ds->dP, enc->n_pad * enc->kw2 * 4));
-    HK(hipMemcpy(ds->dO, enc->O, enc->n_pad * enc->kw1 * 4, hipMemcpyHostToDevice));
-    HK(hipMemcpy(ds->dP, enc->P, enc->n_pad * enc->kw2 * 4, hipMemcpyHostToDevice));
+    HK(upload_planes(enc->O, (int)enc->n_pad, (int)enc->kw1, (int)ds->kw1, &ds->dO));
+    HK(upload_planes(enc->P, (int)enc->n_pad, (int)enc->kw2, (int)ds->kw2, &ds->dP));
     if (enc->n_groups) {
         HK(hipMalloc(&ds->dgmem, enc->n_members * 4));
         HK(hipMalloc(&ds->drstart, ((size_t)enc->n + 1) * 4));
@@ -921,8 +982,8 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
     }
     HK(hipSetDevice(ds->device));
     if (phases & CGH_PHASE_PRODUCT)
-        HK(launch_tiles(resolve_kernel(kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, 0, S,
-                         (n + S - 1) / S, d_out, st));
+        HK(launch_tiles(resolve_kernel(kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
+                         S, (n + S - 1) / S, d_out, st));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         const int chunks = (n - 1 + RCW - 1) / RCW;
         const RareJob jb = {d_out, n, 0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0};
@@ -1005,7 +1066,8 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         b->tile.push_back(tl);
         b->offs.push_back(b->total);
         b->total += cgh_upper_elems(ds->n, S);
-        jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL, ds->dO, ds->dP, ds->n, 0};
+        jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL, ds->dO, ds->dP, ds->n, 0,
+                   (int)ds->n_pad, (int)ds->n_pad};
         const int k = tl == 256 ? 0 : 1;
         const long long cost = (long long)(ds->kw1 + ds->kw2);
         for (int s = 0, r0 = 0; r0 < ds->n; s++, r0 += S) {
@@ -1066,10 +1128,10 @@ hipError_t batch_enqueue(cgh_batch* b, uint16_t* d_out, hipStream_t st, int kern
     for (int k = 0; k < 2; k++) hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, b->dcount + k);
     if (b->ntiles[0]) {
         if (fp4)
-            hipLaunchKernelGGL((mfma_tile_queue<256, 128, 128, true>), dim3(cus), dim3(256), 0, st, b->djobs, d_out,
+            hipLaunchKernelGGL((mfma_tile_queue<256, 128, 64, true>), dim3(cus), dim3(512), 0, st, b->djobs, d_out,
                                b->doffs, b->dtiles[0], b->ntiles[0], b->dcount);
         else
-            hipLaunchKernelGGL((mfma_tile_queue<256, 128, 128, false>), dim3(cus), dim3(256), 0, st, b->djobs, d_out,
+            hipLaunchKernelGGL((mfma_tile_queue<256, 128, 64, false>), dim3(cus), dim3(512), 0, st, b->djobs, d_out,
                                b->doffs, b->dtiles[0], b->ntiles[0], b->dcount);
     }
     if (b->ntiles[1]) {
@@ -1293,7 +1355,8 @@ struct cgh_server {
         std::vector<Request*> reqs;
         int q = 0;
         size_t bytes = 0;
-        uint32_t *dO = NULL, *dP = NULL, *drs = NULL;
+        uint32_t *dO = NULL, *dP = NULL, *drs = NULL;  // dO, dP: query planes
+        uint32_t *dOr = NULL, *dPr = NULL;               // the same rows as encoded (row-major)
         uint64_t* dre = NULL;
         uint16_t *ddist = NULL, *dtopd = NULL;
         uint64_t* dpart = NULL;  // per-chunk nearest-row keys
@@ -1325,7 +1388,7 @@ double now_s() {
 }
 
 size_t batch_bytes(const cgh_db* db, int q) {
-    return (size_t)q * (size_t)db->n * 2 + (size_t)up((size_t)q, 256) * (db->kw1 + db->kw2) * 4 +
+    return (size_t)q * (size_t)db->n * 2 + (size_t)up((size_t)q, 256) * (db->kw1 + db->kw2 + db->ds->kw1 + db->ds->kw2) * 4 +
            (size_t)q * (size_t)db->L * 8 + (size_t)q * TOPK * 6 +
            (size_t)q * (size_t)((db->n + TKC - 1) / TKC) * TOPK * 8;
 }
@@ -1349,13 +1412,16 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     memset(sl.hO + (size_t)q * db->kw1, 0, (size_t)(q_pad - q) * db->kw1 * 4);
     memset(sl.hP + (size_t)q * db->kw2, 0, (size_t)(q_pad - q) * db->kw2 * 4);
     hipError_t e;
-    if ((e = hipMemcpyAsync(sl.dO, sl.hO, (size_t)q_pad * db->kw1 * 4, hipMemcpyHostToDevice, sl.st))) return e;
-    if ((e = hipMemcpyAsync(sl.dP, sl.hP, (size_t)q_pad * db->kw2 * 4, hipMemcpyHostToDevice, sl.st))) return e;
+    if ((e = hipMemcpyAsync(sl.dOr, sl.hO, (size_t)q_pad * db->kw1 * 4, hipMemcpyHostToDevice, sl.st))) return e;
+    if ((e = hipMemcpyAsync(sl.dPr, sl.hP, (size_t)q_pad * db->kw2 * 4, hipMemcpyHostToDevice, sl.st))) return e;
+    if ((e = launch_planes(sl.dOr, q_pad, (int)db->kw1, sl.dO, q_pad, (int)db->ds->kw1, sl.st))) return e;
+    if ((e = launch_planes(sl.dPr, q_pad, (int)db->kw2, sl.dP, q_pad, (int)db->ds->kw2, sl.st))) return e;
     if ((e = hipMemcpyAsync(sl.drs, sl.hrs, (size_t)(q + 1) * 4, hipMemcpyHostToDevice, sl.st))) return e;
     if (ne && (e = hipMemcpyAsync(sl.dre, sl.hre, ne * 8, hipMemcpyHostToDevice, sl.st))) return e;
     const int tile = q > 128 ? 256 : 128;
     TileArgs a;
-    a.job = {sl.dO, sl.dP, (int)db->kw1, (int)db->kw2, q, 0, q, sl.ddist, db->ds->dO, db->ds->dP, db->n, 1};
+    a.job = {sl.dO, sl.dP, (int)db->ds->kw1, (int)db->ds->kw2, q, 0, q, sl.ddist, db->ds->dO, db->ds->dP, db->n, 1,
+             q_pad, (int)db->ds->n_pad};
     a.ns = 1;
     a.q = 0;
     a.nbc = (db->n + tile - 1) / tile;
@@ -1363,8 +1429,8 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     a.chunk = (a.T + 7) / 8;
     const bool fp4 = resolve_kernel(sv->kernel) == CGH_KERNEL_MFMA_FP4;
     const dim3 grid((unsigned)(a.chunk * 8));
-    if (tile == 256 && fp4) hipLaunchKernelGGL((mfma_tile<256, 128, 128, true>), grid, dim3(256), 0, sl.st, a);
-    else if (tile == 256) hipLaunchKernelGGL((mfma_tile<256, 128, 128, false>), grid, dim3(256), 0, sl.st, a);
+    if (tile == 256 && fp4) hipLaunchKernelGGL((mfma_tile<256, 128, 64, true>), grid, dim3(512), 0, sl.st, a);
+    else if (tile == 256) hipLaunchKernelGGL((mfma_tile<256, 128, 64, false>), grid, dim3(512), 0, sl.st, a);
     else if (fp4) hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, sl.st, a);
     else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);
     if (ne) {
@@ -1486,7 +1552,7 @@ extern "C" void cgh_server_free(cgh_server* sv) {
     (void)hipSetDevice(sv->db->device);
     for (auto& sl : sv->slots) {
         if (sl.st) (void)hipStreamSynchronize(sl.st);
-        void* dev[] = {sl.dO, sl.dP, sl.drs, sl.dre, sl.ddist, sl.dtopd, sl.dtopi, sl.dpart};
+        void* dev[] = {sl.dO, sl.dP, sl.dOr, sl.dPr, sl.drs, sl.dre, sl.ddist, sl.dtopd, sl.dtopi, sl.dpart};
         for (void* p : dev)
             if (p) (void)hipFree(p);
         void* host[] = {sl.hO, sl.hP, sl.hrs, sl.hre, sl.hdist, sl.htopd, sl.htopi};
@@ -1537,8 +1603,10 @@ extern "C" int cgh_server_create(cgh_db* db, int slots, int max_batch, size_t hb
         sl.re_cap = (size_t)max_batch * db->L;
         HK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
         HK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
-        HK(hipMalloc(&sl.dO, qp * db->kw1 * 4));
-        HK(hipMalloc(&sl.dP, qp * db->kw2 * 4));
+        HK(hipMalloc(&sl.dOr, qp * db->kw1 * 4));
+        HK(hipMalloc(&sl.dPr, qp * db->kw2 * 4));
+        HK(hipMalloc(&sl.dO, qp * db->ds->kw1 * 4));
+        HK(hipMalloc(&sl.dP, qp * db->ds->kw2 * 4));
         HK(hipMalloc(&sl.drs, ((size_t)max_batch + 1) * 4));
         HK(hipMalloc(&sl.dre, std::max<size_t>(sl.re_cap, 1) * 8));
         HK(hipMalloc(&sl.ddist, up((size_t)max_batch * db->n, 8) * 2));
@@ -1648,7 +1716,8 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
         if (r0 < n) {
             const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
             const size_t w = (size_t)(n - r0);
-            HK(launch_tiles(resolve_kernel(CGH_KERNEL_AUTO), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, n, r0,
+            HK(launch_tiles(resolve_kernel(CGH_KERNEL_AUTO), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2,
+                            (int)ds->n_pad, n, r0,
                             S, 1, dU[b], st[b]));
             if (ds->n_rare && n - r0 > 1) {
                 const int chunks = (n - r0 - 1 + RCW - 1) / RCW;
