@@ -478,46 +478,46 @@ sub_rare_csr(uint16_t* out, int w, int r0, int rows, const uint32_t* start, cons
 // loads, every rare partner j of row i is subtracted there with an LDS
 // atomic, and the chunk is written back: HBM sees each output cell read and
 // written once and the scattered updates stay on-chip.  Row i's partners are
-// ranges of the ascending group member lists (rentry); a batch of up to RNT
-// ranges is clipped to the chunk and prefix-summed in LDS, then the block
-// walks the flattened partner list with each lane taking every 64th partner
-// of its wavefront's span, so consecutive lanes read consecutive members and
-// RUN loads are in flight per lane.
+// ranges of the ascending group member lists (rentry).  A batch of up to RNT
+// ranges is read (and clipped to the chunk) by all threads at once into LDS;
+// then each 32-lane subgroup takes ranges from LDS, its lanes reading
+// consecutive members (RUN loads in flight per lane) and applying them.
 constexpr int RCW = 65536;  // u16 cells per chunk (128 KiB of LDS: one workgroup per CU)
 constexpr int RNT = 1024;   // threads per workgroup = ranges per batch
 constexpr int RQ = (RCW / 8 + 1 + RNT - 1) / RNT;  // 16-byte words staged per thread
 constexpr int RUN = 8;      // partners per lane per pass
+constexpr int RSG = 32;     // lanes per range
 struct RareJob {
     uint16_t* out;             // stripe layout of rows r0..
     int n, r0, S, chunks;      // chunks = column chunks per row
     const uint32_t* rstart;    // n + 1 offsets into rentry
     const uint64_t* rentry;    // partner ranges of gmem
-    const uint32_t* gmem;      // group members
+    const void* gmem;          // group members: u16 if m16 (every row < 65536), else u32
     long long block0;          // first workgroup of this job in a batched launch
     int d;                     // dataset of this job in a batched launch
     int rect, nB;              // rectangular form: rows against nB columns
+    int m16;
 };
 
 struct RareShared {
     uint4 cells4[RCW / 8 + 1];
-    uint32_t ebeg[RNT];
-    uint32_t epre[RNT + 1];
-    uint32_t wsum[RNT / 64];
+    uint32_t ebeg[RNT];  // batch ranges [ebeg, eend) of member indices
+    uint32_t eend[RNT];
 };
 
-// Row i (>= r0), column chunk ch of job jb.
-__device__ __forceinline__ void rare_rows_body(const RareJob& jb, int i, int ch, RareShared& sh) {
+// Row i (>= r0), column chunk ch of job jb, members of type M.
+template <typename M>
+__device__ __forceinline__ void rare_rows_impl(const RareJob& jb, const M* __restrict__ gmem, int i, int ch,
+                                               RareShared& sh) {
     uint16_t* __restrict__ out = jb.out;
     const int n = jb.n, r0 = jb.r0, S = jb.S, chunks = jb.chunks;
     const uint32_t* __restrict__ rstart = jb.rstart;
     const uint64_t* __restrict__ rentry = jb.rentry;
-    const uint32_t* __restrict__ gmem = jb.gmem;
     uint4* cells4 = sh.cells4;
     uint32_t* ebeg = sh.ebeg;
-    uint32_t* epre = sh.epre;
-    uint32_t* wsum = sh.wsum;
+    uint32_t* eend = sh.eend;
     uint32_t* cells = reinterpret_cast<uint32_t*>(cells4);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
     // symmetric: cells j in (i, n) of the stripe layout of rows r0..
     // (stripe s = rows r0+s*S ..); rectangular: cells j in [0, nB) of row
@@ -536,10 +536,7 @@ __device__ __forceinline__ void rare_rows_body(const RareJob& jb, int i, int ch,
     {  // every staging load in flight before the first LDS write
         uint4 v[RQ];
 #pragma unroll
-        for (int u = 0; u < RQ; u++) {
-            const int k = tid + u * RNT;
-            if (k < nq) v[u] = src[k];
-        }
+        for (int u = 0; u < RQ; u++) v[u] = src[min(tid + u * RNT, nq - 1)];  // defined: v stays in registers
 #pragma unroll
         for (int u = 0; u < RQ; u++) {
             const int k = tid + u * RNT;
@@ -549,7 +546,6 @@ __device__ __forceinline__ void rare_rows_body(const RareJob& jb, int i, int ch,
 
     for (uint32_t eb = e0; eb < e1; eb += RNT) {
         const int nb = (int)min((uint32_t)RNT, e1 - eb);
-        uint32_t len = 0;
         if (tid < nb) {
             const uint64_t r = rentry[eb + tid];
             uint32_t b = (uint32_t)r, end = (uint32_t)(r >> 32);
@@ -568,56 +564,25 @@ __device__ __forceinline__ void rare_rows_body(const RareJob& jb, int i, int ch,
                 end = lo;
             }
             ebeg[tid] = b;
-            len = end - b;
+            eend[tid] = end;
         }
-        // block-wide inclusive scan of len -> epre[1..nb], epre[0] = 0
-        uint32_t v = len;
+        __syncthreads();  // also orders the staging stores before the atomics
+        const int sub = tid / RSG, sl = tid % RSG;
+        for (int e = sub; e < nb; e += RNT / RSG) {
+            const uint32_t b = ebeg[e], end = eend[e];
+            for (uint32_t p = b + sl; p < end; p += RSG * RUN) {
+                uint32_t jj[RUN];
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t y = __shfl_up(v, d, 64);
-            if (lane >= d) v += y;
-        }
-        if (lane == 63) wsum[wave] = v;
-        __syncthreads();  // also orders the staging loads before the atomics
-        uint32_t off = 0;
-        for (int w = 0; w < wave; w++) off += wsum[w];
-        epre[tid + 1] = off + v;
-        if (tid == 0) epre[0] = 0;
-        __syncthreads();
-        const uint32_t P = epre[nb];
-        // wavefront w takes spans [w*64*RUN + t*RNT*RUN, +64*RUN)
-        for (uint32_t span = (uint32_t)wave * 64 * RUN; span < P; span += RNT * RUN) {
-            uint32_t q = span + lane;
-            int e = 0;
-            if (q < P) {
-                int lo = 0, hi = nb - 1;  // last e with epre[e] <= q
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (epre[mid] <= q) lo = mid; else hi = mid - 1;
-                }
-                e = lo;
+                for (int u = 0; u < RUN; u++) jj[u] = p + u * RSG < end ? (uint32_t)gmem[p + u * RSG] : 0xffffffffu;
+#pragma unroll
+                for (int u = 0; u < RUN; u++)
+                    if (jj[u] != 0xffffffffu) {
+                        const size_t o = base + (size_t)jj[u] - a0;  // u16 offset in LDS
+                        atomicSub(&cells[o >> 1], (o & 1) ? 0x10000u : 1u);
+                    }
             }
-            uint32_t idx[RUN];
-            bool ok[RUN];
-#pragma unroll
-            for (int u = 0; u < RUN; u++, q += 64) {
-                ok[u] = q < P;
-                if (ok[u]) {
-                    while (epre[e + 1] <= q) e++;
-                    idx[u] = ebeg[e] + (q - epre[e]);
-                }
-            }
-            uint32_t jj[RUN];
-#pragma unroll
-            for (int u = 0; u < RUN; u++) jj[u] = ok[u] ? gmem[idx[u]] : 0;
-#pragma unroll
-            for (int u = 0; u < RUN; u++)
-                if (ok[u]) {
-                    const size_t o = base + (size_t)jj[u] - a0;  // u16 offset in LDS
-                    atomicSub(&cells[o >> 1], (o & 1) ? 0x10000u : 1u);
-                }
         }
-        __syncthreads();  // ebeg / epre are rewritten by the next batch
+        __syncthreads();  // ebeg / eend are rewritten by the next batch
     }
 
     uint4* __restrict__ dst = reinterpret_cast<uint4*>(out + a0);
@@ -631,6 +596,13 @@ __device__ __forceinline__ void rare_rows_body(const RareJob& jb, int i, int ch,
                 if (lo + h >= g0 && lo + h < g1) out[lo + h] = c16[h];
         }
     }
+}
+
+// Member lists are read as u16 when every row index fits: half the bytes of
+// the pass's dominant (L2-missing) reads.
+__device__ __forceinline__ void rare_rows_body(const RareJob& jb, int i, int ch, RareShared& sh) {
+    if (jb.m16) rare_rows_impl(jb, static_cast<const uint16_t*>(jb.gmem), i, ch, sh);
+    else rare_rows_impl(jb, static_cast<const uint32_t*>(jb.gmem), i, ch, sh);
 }
 
 // Rows r0 .. r0 + gridDim.x / chunks - 1 of one job.
@@ -727,6 +699,27 @@ done:
 }
 
 int kc_words(size_t kw) { return (int)up(kw, (size_t)KC); }
+
+// Member array (u32 row indices) -> a new device buffer, narrowed to u16
+// when m16.
+hipError_t upload_members(const uint32_t* mem, size_t count, bool m16, void** out) {
+    hipError_t e;
+    *out = NULL;
+    if ((e = hipMalloc(out, std::max<size_t>(count, 1) * (m16 ? 2 : 4)))) return e;
+    if (!count) return hipSuccess;
+    if (m16) {
+        std::vector<uint16_t> h(count);
+        for (size_t k = 0; k < count; k++) h[k] = (uint16_t)mem[k];
+        e = hipMemcpy(*out, h.data(), count * 2, hipMemcpyHostToDevice);
+    } else {
+        e = hipMemcpy(*out, mem, count * 4, hipMemcpyHostToDevice);
+    }
+    if (e) {
+        (void)hipFree(*out);
+        *out = NULL;
+    }
+    return e;
+}
 
 hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int ld, int n,
                         int r0, int S, int ns, uint16_t* out, hipStream_t st) {
@@ -894,7 +887,9 @@ struct cgh_dataset {
     size_t n_groups, n_members;
     uint64_t n_rare;
     size_t n_rentries;
-    uint32_t *dgmem, *drstart;
+    void* dgmem;              // members, u16 when m16
+    int m16;
+    uint32_t* drstart;
     uint64_t* drentry;
 };
 
@@ -935,10 +930,10 @@ extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_datas
     HK(upload_planes(enc->O, (int)enc->n_pad, (int)enc->kw1, (int)ds->kw1, &ds->dO));
     HK(upload_planes(enc->P, (int)enc->n_pad, (int)enc->kw2, (int)ds->kw2, &ds->dP));
     if (enc->n_groups) {
-        HK(hipMalloc(&ds->dgmem, enc->n_members * 4));
+        ds->m16 = enc->n <= 65536;
+        HK(upload_members(enc->gmem, enc->n_members, ds->m16, &ds->dgmem));
         HK(hipMalloc(&ds->drstart, ((size_t)enc->n + 1) * 4));
         HK(hipMalloc(&ds->drentry, std::max<size_t>(enc->n_rentries, 1) * 8));
-        HK(hipMemcpy(ds->dgmem, enc->gmem, enc->n_members * 4, hipMemcpyHostToDevice));
         HK(hipMemcpy(ds->drstart, enc->rstart, ((size_t)enc->n + 1) * 4, hipMemcpyHostToDevice));
         if (enc->n_rentries)
             HK(hipMemcpy(ds->drentry, enc->rentry, enc->n_rentries * 8, hipMemcpyHostToDevice));
@@ -986,7 +981,7 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
                          S, (n + S - 1) / S, d_out, st));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         const int chunks = (n - 1 + RCW - 1) / RCW;
-        const RareJob jb = {d_out, n, 0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0};
+        const RareJob jb = {d_out, n, 0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16};
         hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)n * chunks)), dim3(RNT), 0, st, jb);
         HK(hipGetLastError());
     }
@@ -1077,7 +1072,7 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         }
         if (ds->n_rare && ds->n > 1) {
             RareJob r = {NULL, ds->n, 0, S, (ds->n - 1 + RCW - 1) / RCW, ds->drstart, ds->drentry, ds->dgmem,
-                         b->rare_blocks, d};
+                         b->rare_blocks, d, 0, 0, ds->m16};
             rjobs.push_back(r);
             b->rare_blocks += (long long)ds->n * r.chunks;
         }
@@ -1293,7 +1288,8 @@ struct cgh_db {
     size_t kw1 = 0, kw2 = 0, D = 0;
     cgh_codec* codec = NULL;    // host dictionaries: query ids -> database columns and rare lists
     cgh_dataset* ds = NULL;     // O, P of the database on the device
-    uint32_t* dqmem = NULL;     // the codec's member array on the device
+    void* dqmem = NULL;         // the codec's member array on the device (u16 when m16)
+    int m16 = 0;
 };
 
 extern "C" void cgh_db_free(cgh_db* db) {
@@ -1318,8 +1314,8 @@ extern "C" int cgh_db_create(int device, const uint64_t* calls, int n, int L, in
     if (cgh_dataset_upload(device, &enc, &db->ds, err, errlen)) goto fail;
     cgh_encoding_free(&enc);
     HK(hipSetDevice(device));
-    HK(hipMalloc(&db->dqmem, std::max<size_t>(nmem, 1) * 4));
-    if (nmem) HK(hipMemcpy(db->dqmem, cgh_codec_members(db->codec), nmem * 4, hipMemcpyHostToDevice));
+    db->m16 = db->n <= 65536;
+    HK(upload_members(cgh_codec_members(db->codec), nmem, db->m16, &db->dqmem));
     *out = db;
     return 0;
 fail:
@@ -1435,7 +1431,7 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);
     if (ne) {
         const int chunks = (db->n + RCW - 1) / RCW;
-        const RareJob jb = {sl.ddist, q, 0, q, chunks, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n};
+        const RareJob jb = {sl.ddist, q, 0, q, chunks, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n, db->m16};
         hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)q * chunks)), dim3(RNT), 0, sl.st, jb);
     }
     const int tchunks = (db->n + TKC - 1) / TKC;
@@ -1721,7 +1717,8 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
                             S, 1, dU[b], st[b]));
             if (ds->n_rare && n - r0 > 1) {
                 const int chunks = (n - r0 - 1 + RCW - 1) / RCW;
-                const RareJob jb = {dU[b], n, r0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0};
+                const RareJob jb = {dU[b], n, r0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0,
+                                    ds->m16};
                 hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)rows * chunks)), dim3(RNT), 0, st[b], jb);
                 HK(hipGetLastError());
             }
