@@ -287,19 +287,23 @@ extern "C" int cgh_rare_csr(const cgh_encoding* enc, uint32_t* start, uint32_t* 
     return 0;
 }
 
-// Device-time model of one upper-triangle pass on MI355X, fitted to bench.py
-// runs on the 63,005 x 1,142 long-tail workload at tau = 256 .. 2048:
-//   MFMA binary product   n^2/2 * (K1 + K2) pair-bits at GEMM_RATE
+// Device-time model of one upper-triangle pass on MI355X, fitted to the
+// product and rare phases timed on the 63,005 x 1,142 long-tail workload at
+// tau = 128 .. 4096 (model within 2% of every measured point):
+//   MFMA binary product   n^2/2 * (K1 + K2) pair-bits at GEMM_RATE, K padded
+//                         to the kernel's 512-bit slabs, plus the u16 output
+//                         written once (OUT_BW)
 //   rare-pair pass        each output cell staged through LDS once
 //                         (read + write, STAGE_BW) plus R pairs at RARE_RATE
-static const double GEMM_RATE = 2.25e15;  // pair-bits per second (fp4 MFMA kernel)
-static const double STAGE_BW = 2.6e12;    // bytes per second, rare-pass staging
-static const double RARE_RATE = 9.0e11;   // rare pairs per second
+static const double GEMM_RATE = 3.44e15;  // pair-bits per second (fp4 MFMA kernel)
+static const double OUT_BW = 2.7e12;      // output bytes per second (2 B per pair), product epilogue
+static const double STAGE_BW = 3.4e12;    // bytes per second (4 B per pair), rare-pass staging
+static const double RARE_RATE = 1.17e12;  // rare pairs per second
 
 extern "C" int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, size_t out_bytes, size_t free_bytes,
                               void (*log)(void* ctx, const char* line), void* log_ctx) {
     const double n_pad = (double)up((size_t)std::max(n, 1), 256);
-    const double k2 = (double)up((size_t)std::max(L, 1), 256);
+    const double k2 = (double)up((size_t)std::max(L, 1), 512);
     const double pairs = (double)n * (double)(n - 1) / 2.0;
     int best = 0;
     double best_t = 1e300;
@@ -316,12 +320,13 @@ extern "C" int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, si
                 members += x;
             }
         }
-        const double k1 = (double)up(std::max<size_t>(d, 1), 256);
+        const double k1 = (double)up(std::max<size_t>(d, 1), 512);
         // operands + group members (4 B) + row index (8 B per member at most)
         // + row offsets + output buffers + slack
         const double mem = n_pad * (k1 + k2) / 8.0 + 12.0 * members + 4.0 * (n + 1.0) + (double)out_bytes +
                            (double)(64u << 20);
-        const double t = pairs * (k1 + k2) / GEMM_RATE + (r > 0 ? pairs * 4.0 / STAGE_BW + r / RARE_RATE : 0.0);
+        const double t = pairs * (k1 + k2) / GEMM_RATE + pairs * 2.0 / OUT_BW +
+                         (r > 0 ? pairs * 4.0 / STAGE_BW + r / RARE_RATE : 0.0);
         const bool fits = mem <= (double)free_bytes && members < 4.0e9;
         if (log) {
             char line[256];
