@@ -19,9 +19,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "cgmlst-hamming-gpu_amd")
 
 N, L, S = 63005, 1142, 4096
-# MI355X published dense MFMA peaks per product kernel (CGH_KERNEL_*)
+# MI355X published peaks per product kernel (CGH_KERNEL_*), in the units of
+# the algorithmic count 2*(D+L) ops per pair.  The VALU form has no published
+# popcount figure: its peak is derived from the published 157.3 TFLOP/s vector
+# fp32 rate (78.6 T lane-instructions/s), one AND + one v_bcnt per 32 bit
+# pairs = 32 ops per lane-instruction.
 PEAKS = {1: (5000.0, "TOP/s", "int8 (0/1 operands, int32 accumulate)"),
-         2: (10100.0, "TFLOP/s", "fp4 e2m1 (0/+-1 operands, fp32 accumulate of integers <= L: exact)")}
+         2: (10100.0, "TFLOP/s", "fp4 e2m1 (0/+-1 operands, fp32 accumulate of integers <= L: exact)"),
+         3: (2517.0, "TOP/s", "u32 (VALU AND + popcount, u32 accumulate)")}
+MEASURED_PEAK_KEY = {1: "int8_mfma_tops", 2: "fp4_mfma_tflops", 3: "valu_popcount_tops"}
 
 
 def log(msg):
@@ -38,6 +44,56 @@ def load_package():
     sys.modules[name] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def kernel_source_sha():
+    """sha256 of the kernel source the library was built from."""
+    import hashlib
+    with open(os.path.join(PKG_DIR, "csrc", "hamming.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def summarize_counters(dirs, tau, kern, out_path):
+    """FETCH_SIZE / WRITE_SIZE passes of the headline and of --copy-calibration
+    (rocprofv3 --pmc ... --output-format csv, one directory each) -> HBM bytes
+    per launch of the product and rare kernels, scaled by what the same
+    counter reads for the 1 GiB copy, written as JSON for the bench line."""
+    import csv
+    from collections import defaultdict
+
+    def per_dispatch(d, counter):
+        vals = defaultdict(float)
+        names = {}
+        with open(os.path.join(d, "run_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    vals[r["Dispatch_Id"]] += float(r["Counter_Value"])
+                    names[r["Dispatch_Id"]] = r["Kernel_Name"]
+        return [(names[k], v) for k, v in vals.items()]
+
+    def kernel_avg(rows, key):
+        v = [x for name, x in rows if key in name]
+        return sum(v) / len(v) if v else None
+
+    fetch, write, cfetch, cwrite = dirs
+    copy_bytes = 1 << 30
+    cf = max(x for _, x in per_dispatch(cfetch, "FETCH_SIZE"))
+    cw = max(x for _, x in per_dispatch(cwrite, "WRITE_SIZE"))
+    ff, wf = copy_bytes / 1024 / cf, copy_bytes / 1024 / cw
+    rec = {"workload": "long_tail", "n": N, "L": L, "tau": tau, "stripe": S, "product_kernel": kern,
+           "kernel_source_sha256": kernel_source_sha(),
+           "calibration": {"copy_bytes": copy_bytes, "FETCH_SIZE_KiB": cf, "WRITE_SIZE_KiB": cw,
+                           "fetch_factor": ff, "write_factor": wf},
+           "kernels": {}}
+    for label, key in (("product", "valu_tile" if kern == 3 else "mfma_tile"), ("rare", "sub_rare_rows")):
+        fk = kernel_avg(per_dispatch(fetch, "FETCH_SIZE"), key)
+        wk = kernel_avg(per_dispatch(write, "WRITE_SIZE"), key)
+        rec["kernels"][label] = {"name_match": key, "FETCH_SIZE_KiB": fk, "WRITE_SIZE_KiB": wk,
+                                 "fetch_bytes": fk * 1024 * ff, "write_bytes": wk * 1024 * wf,
+                                 "traffic_bytes": fk * 1024 * ff + wk * 1024 * wf}
+    with open(out_path, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(json.dumps(rec["kernels"]), flush=True)
 
 
 def cpu_baseline(X, flat, cgh, target_s=15.0):
@@ -227,7 +283,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--tau", type=int, default=0, help="0 = cost model")
-    ap.add_argument("--kernel", type=int, default=0, help="CGH_KERNEL_*: 0 auto, 1 int8 MFMA, 2 fp4 MFMA")
+    ap.add_argument("--kernel", type=int, default=0,
+                    help="CGH_KERNEL_*: 0 auto, 1 int8 MFMA, 2 fp4 MFMA, 3 VALU AND + popcount")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration) and exit")
@@ -235,7 +292,13 @@ def main():
                     help="long_tail: the 63,005 x 1,142 headline; batch: 64 uneven datasets, batched vs plain; "
                          "serve: resident 200k database, Poisson query stream")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
+    ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="no GPU: turn four rocprofv3 --pmc csv directories into "
+                         "profiles/hbm_traffic_long_tail_63005x1142.json (needs --tau)")
     args = ap.parse_args()
+    if args.summarize_counters:
+        return summarize_counters(args.summarize_counters, args.tau, 2 if args.kernel == 0 else args.kernel,
+                                  os.path.join(ROOT, "profiles", "hbm_traffic_long_tail_63005x1142.json"))
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the distance path runs on a single device "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
@@ -311,12 +374,14 @@ def main():
 
     # HBM bytes per launch of the product kernel, from the committed counter
     # passes (profiles/hbm_traffic_*.json) when they cover this configuration
+    # and were taken on this kernel source
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_long_tail_63005x1142.json")
     if os.path.exists(tpath):
         t_rec = json.load(open(tpath))
-        if (t_rec["n"], t_rec["L"], t_rec["tau"], t_rec["stripe"]) == (N, L, tau, S):
-            traffic = t_rec["kernels"]["mfma_tile"]["traffic_bytes"]
+        if ((t_rec["n"], t_rec["L"], t_rec["tau"], t_rec["stripe"], t_rec.get("product_kernel")) == (N, L, tau, S, kern)
+                and t_rec.get("kernel_source_sha256") == kernel_source_sha()):
+            traffic = t_rec["kernels"]["product"]["traffic_bytes"]
 
     measured = cgh.measure_peaks(0)
     log("measured peaks: " + ", ".join(f"{k} {v:.0f}" for k, v in measured.items()))
@@ -342,13 +407,14 @@ def main():
         "data": "synthetic (seeded long-tail Zipf(1.1) allele calls, Beta(0.5,15) per-sample missing)",
         "config": {"workload": "long_tail", "n": N, "L": L, "tau": tau, "stripe": S, "layout": "upper-tri u16",
                    "one_hot_cols": D, "rare_pairs": R, "tile": ds.tile(), "kernel": kern},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": punit,
+        "roofline": {"bound": "valu" if kern == 3 else "mfma", "achieved": achieved, "peak": peak, "unit": punit,
                      "frac": achieved / peak, "traffic": traffic,
-                     "kernel": "mfma_tile (binary product + u16 epilogue, one launch per step)",
+                     "kernel": ("valu_tile" if kern == 3 else "mfma_tile") +
+                               " (binary product + u16 epilogue, one launch per step)",
                      "work": "2*(D+L) ops per pair (multiply-add per bit pair), pairs = n(n-1)/2"},
         "phases_ms": {"product": prod_ms, "rare": rare_ms},
         "measured_peaks": measured,
-        "frac_of_measured_peak": achieved / (measured["fp4_mfma_tflops"] if kern == 2 else measured["int8_mfma_tops"]),
+        "frac_of_measured_peak": achieved / measured[MEASURED_PEAK_KEY[kern]],
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

@@ -452,6 +452,183 @@ mfma_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* 
     }
 }
 
+// ---------------------------------------------------------------------------
+// VALU form of the binary product: AND + v_bcnt_u32_b32 on 32-bit words
+// ---------------------------------------------------------------------------
+//
+// 128 x 128 sub-tiles by 256 threads, 8 x 8 cells per thread (rows ty + 16 i,
+// columns tx + 16 j).  Each KC-word slab of the sub-tile's A and B rows is
+// loaded with 16-byte reads (4 per thread, planes layout: consecutive threads
+// read consecutive rows) into a double-buffered LDS stage as [row][4 x 16 B];
+// a thread reads 4 words of each of its 16 rows with one ds_read_b128 and
+// accumulates popcount(a & b) per cell.  The O slabs come first; the
+// accumulator is negated before the first P slab, so it ends as
+// P P^T - O O^T.  Tiles of 256 are done as 2 x 2 sub-tiles.
+constexpr int VB = 128;
+struct ValuStage {
+    uint4 a[VB][KC / 4];
+    uint4 b[VB][KC / 4];
+};
+
+__device__ __forceinline__ void valu_sub(const TileJob a, int sr0, int rt0, int ct0, int rows, int w,
+                                         uint16_t* __restrict__ out, ValuStage (&st)[2]) {
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int nc1 = a.kw1 / KC, nc = nc1 + a.kw2 / KC;
+    const int growA = sr0 + rt0, growB = a.rect ? ct0 : sr0 + ct0;
+    uint32_t acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[i][j] = 0;
+    // piece q = tid + 256 u: side q / 512, plane q / 128 % 4, row q % 128.
+    // Rows past the valid ones read the last valid row (never stored).
+    size_t off[4], pitch[4];
+    int dst_side[4], dst_pl[4], dst_r[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int q = tid + 256 * u;
+        dst_side[u] = q >> 9;
+        dst_pl[u] = (q >> 7) & 3;
+        dst_r[u] = q & 127;
+        const int grow = dst_side[u] ? min(growB + dst_r[u], a.nB - 1) : min(growA + dst_r[u], a.n - 1);
+        pitch[u] = (size_t)(dst_side[u] ? a.ldB : a.ldA);
+        off[u] = ((size_t)dst_pl[u] * pitch[u] + grow) * 4;  // words, from the slab's first plane
+    }
+    // slab c of the O or P planes: base pointer and first plane
+#define VALU_PIECE(c, u)                                                                                       \
+    (*reinterpret_cast<const uint4*>(                                                                         \
+        ((c) < nc1 ? (dst_side[u] ? a.OB : a.O) : (dst_side[u] ? a.PB : a.P)) +                                 \
+        ((size_t)(((c) < nc1 ? (c) : (c) - nc1) * (KC / 4)) * pitch[u]) * 4 + off[u]))
+    uint4 v0 = VALU_PIECE(0, 0), v1 = VALU_PIECE(0, 1), v2 = VALU_PIECE(0, 2), v3 = VALU_PIECE(0, 3);
+#define VALU_STORE(buf, u, v)                                      \
+    do {                                                           \
+        if (dst_side[u]) st[buf].b[dst_r[u]][dst_pl[u]] = (v);     \
+        else st[buf].a[dst_r[u]][dst_pl[u]] = (v);                 \
+    } while (0)
+    VALU_STORE(0, 0, v0);
+    VALU_STORE(0, 1, v1);
+    VALU_STORE(0, 2, v2);
+    VALU_STORE(0, 3, v3);
+    __syncthreads();
+    for (int c = 0; c < nc; c++) {
+        const int buf = c & 1;
+        if (c + 1 < nc) {  // next slab in flight during this one
+            v0 = VALU_PIECE(c + 1, 0);
+            v1 = VALU_PIECE(c + 1, 1);
+            v2 = VALU_PIECE(c + 1, 2);
+            v3 = VALU_PIECE(c + 1, 3);
+        }
+        if (c == nc1) {
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) acc[i][j] = 0u - acc[i][j];
+        }
+#pragma unroll 1
+        for (int pl = 0; pl < KC / 4; pl++) {  // not unrolled: 16 rows x 4 words live at a time
+            uint4 ar[8], br[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) ar[i] = st[buf].a[ty + 16 * i][pl];
+#pragma unroll
+            for (int j = 0; j < 8; j++) br[j] = st[buf].b[tx + 16 * j][pl];
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    uint32_t t = acc[i][j];
+                    t += __builtin_popcount(ar[i].x & br[j].x);
+                    t += __builtin_popcount(ar[i].y & br[j].y);
+                    t += __builtin_popcount(ar[i].z & br[j].z);
+                    t += __builtin_popcount(ar[i].w & br[j].w);
+                    acc[i][j] = t;
+                }
+        }
+        if (c + 1 < nc) {  // that stage was last read before the previous barrier
+            VALU_STORE(buf ^ 1, 0, v0);
+            VALU_STORE(buf ^ 1, 1, v1);
+            VALU_STORE(buf ^ 1, 2, v2);
+            VALU_STORE(buf ^ 1, 3, v3);
+        }
+        __syncthreads();
+    }
+#undef VALU_PIECE
+#undef VALU_STORE
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int r = rt0 + ty + 16 * i;
+        if (r >= rows) continue;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int cc = ct0 + tx + 16 * j;
+            if (cc < w) out[(size_t)r * w + cc] = (uint16_t)(a.rect || cc > r ? acc[i][j] : 0u);
+        }
+    }
+}
+
+// One BM x BM tile (I, J) of stripe s of job a (BM = 128 or 256).
+template <int BM>
+__device__ __forceinline__ void valu_body(const TileJob a, int s, int I, int J, ValuStage (&st)[2]) {
+    const int sr0 = a.r0 + s * a.S;
+    const int rows = min(a.S, a.n - sr0), w = a.rect ? a.nB : a.n - sr0;
+    uint16_t* __restrict__ out = a.rect ? a.out + (size_t)(sr0 - a.r0) * a.nB
+                                        : a.out + stripe_off((size_t)(a.n - a.r0), (size_t)a.S, (size_t)s);
+    const int rt0 = I * BM, ct0 = J * BM;
+    if (!a.rect && J < I) {  // entirely below the diagonal
+        const int rend = min(BM, rows - rt0), cend = min(BM, w - ct0);
+        for (int e = threadIdx.x; e < rend * BM; e += 256) {
+            const int r = e / BM, c = e % BM;
+            if (c < cend) out[(size_t)(rt0 + r) * w + ct0 + c] = 0;
+        }
+        return;
+    }
+    for (int di = 0; di < BM / VB; di++)
+        for (int dj = 0; dj < BM / VB; dj++) {
+            const int r0 = rt0 + di * VB, c0 = ct0 + dj * VB;
+            if (r0 >= rows || c0 >= w) continue;
+            valu_sub(a, sr0, r0, c0, rows, w, out, st);
+            __syncthreads();  // the stages are reused by the next sub-tile
+        }
+}
+
+template <int BM>
+__global__ void __launch_bounds__(256) valu_tile(TileArgs a) {
+    __shared__ ValuStage st[2];
+    const long long t = (long long)(blockIdx.x & 7) * a.chunk + (blockIdx.x >> 3);
+    if (t >= a.T) return;
+    int s = 0;
+    if (a.ns > 1) {
+        int lo = 0, hi = a.ns - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tiles_before(mid, a.q, a.nbc) <= t) lo = mid; else hi = mid - 1;
+        }
+        s = lo;
+    }
+    const long long local = t - (s ? tiles_before(s, a.q, a.nbc) : 0);
+    const int tn = a.nbc - s * a.q;
+    valu_body<BM>(a.job, s, (int)(local / tn), (int)(local % tn), st);
+}
+
+template <int BM>
+__global__ void __launch_bounds__(256)
+valu_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* __restrict__ outoff,
+                const int4* __restrict__ tiles, int T, int* __restrict__ next) {
+    __shared__ ValuStage st[2];
+    __shared__ int slot;
+    for (;;) {
+        if (threadIdx.x == 0) slot = atomicAdd(next, 1);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(slot);
+        __syncthreads();
+        if (t >= T) break;
+        const int4 tl = tiles[t];
+        TileJob jb = jobs[tl.x];
+        jb.out = base + outoff[tl.x];
+        scalarize(jb);
+        valu_body<BM>(jb, tl.y, tl.z, tl.w, st);
+    }
+}
+
 __global__ void reset_counter(int* c) { *c = 0; }
 
 // Subtract one from the u16 cell at element offset off of a 4-byte aligned
@@ -745,7 +922,10 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
     a.chunk = (a.T + 7) / 8;
     const dim3 grid((unsigned)(a.chunk * 8));
     const bool fp4 = kernel == CGH_KERNEL_MFMA_FP4;
-    if (tile == 256 && fp4)
+    if (kernel == CGH_KERNEL_VALU_POPC) {
+        if (tile == 256) hipLaunchKernelGGL((valu_tile<256>), grid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((valu_tile<128>), grid, dim3(256), 0, st, a);
+    } else if (tile == 256 && fp4)
         hipLaunchKernelGGL((mfma_tile<256, 128, 64, true>), grid, dim3(512), 0, st, a);
     else if (tile == 256)
         hipLaunchKernelGGL((mfma_tile<256, 128, 64, false>), grid, dim3(512), 0, st, a);
@@ -759,7 +939,8 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
 // CGH_KERNEL_AUTO resolves to the fastest measured product kernel.
 int resolve_kernel(int kernel) { return kernel == CGH_KERNEL_AUTO ? CGH_KERNEL_MFMA_FP4 : kernel; }
 bool valid_kernel(int kernel) {
-    return kernel == CGH_KERNEL_AUTO || kernel == CGH_KERNEL_MFMA_I8 || kernel == CGH_KERNEL_MFMA_FP4;
+    return kernel == CGH_KERNEL_AUTO || kernel == CGH_KERNEL_MFMA_I8 || kernel == CGH_KERNEL_MFMA_FP4 ||
+           kernel == CGH_KERNEL_VALU_POPC;
 }
 
 }  // namespace
@@ -1121,8 +1302,12 @@ hipError_t batch_enqueue(cgh_batch* b, uint16_t* d_out, hipStream_t st, int kern
     const bool fp4 = resolve_kernel(kernel) == CGH_KERNEL_MFMA_FP4;
     const unsigned cus = (unsigned)b->cus;
     for (int k = 0; k < 2; k++) hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, b->dcount + k);
+    const bool valu = resolve_kernel(kernel) == CGH_KERNEL_VALU_POPC;
     if (b->ntiles[0]) {
-        if (fp4)
+        if (valu)
+            hipLaunchKernelGGL((valu_tile_queue<256>), dim3(cus * 2), dim3(256), 0, st, b->djobs, d_out, b->doffs,
+                               b->dtiles[0], b->ntiles[0], b->dcount);
+        else if (fp4)
             hipLaunchKernelGGL((mfma_tile_queue<256, 128, 64, true>), dim3(cus), dim3(512), 0, st, b->djobs, d_out,
                                b->doffs, b->dtiles[0], b->ntiles[0], b->dcount);
         else
@@ -1130,7 +1315,10 @@ hipError_t batch_enqueue(cgh_batch* b, uint16_t* d_out, hipStream_t st, int kern
                                b->doffs, b->dtiles[0], b->ntiles[0], b->dcount);
     }
     if (b->ntiles[1]) {
-        if (fp4)
+        if (valu)
+            hipLaunchKernelGGL((valu_tile_queue<128>), dim3(cus * 2), dim3(256), 0, st, b->djobs, d_out, b->doffs,
+                               b->dtiles[1], b->ntiles[1], b->dcount + 1);
+        else if (fp4)
             hipLaunchKernelGGL((mfma_tile_queue<128, 64, 64, true>), dim3(cus * 2), dim3(256), 0, st, b->djobs,
                                d_out, b->doffs, b->dtiles[1], b->ntiles[1], b->dcount + 1);
         else
@@ -1425,7 +1613,11 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     a.chunk = (a.T + 7) / 8;
     const bool fp4 = resolve_kernel(sv->kernel) == CGH_KERNEL_MFMA_FP4;
     const dim3 grid((unsigned)(a.chunk * 8));
-    if (tile == 256 && fp4) hipLaunchKernelGGL((mfma_tile<256, 128, 64, true>), grid, dim3(512), 0, sl.st, a);
+    if (sv->kernel == CGH_KERNEL_VALU_POPC && tile == 256)
+        hipLaunchKernelGGL((valu_tile<256>), grid, dim3(256), 0, sl.st, a);
+    else if (sv->kernel == CGH_KERNEL_VALU_POPC)
+        hipLaunchKernelGGL((valu_tile<128>), grid, dim3(256), 0, sl.st, a);
+    else if (tile == 256 && fp4) hipLaunchKernelGGL((mfma_tile<256, 128, 64, true>), grid, dim3(512), 0, sl.st, a);
     else if (tile == 256) hipLaunchKernelGGL((mfma_tile<256, 128, 64, false>), grid, dim3(512), 0, sl.st, a);
     else if (fp4) hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, sl.st, a);
     else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);

@@ -147,6 +147,7 @@ size_t cgh_stripe_offset(int n, int S, int s);
 #define CGH_KERNEL_AUTO 0     /* the faster one on MI355X (fp4) */
 #define CGH_KERNEL_MFMA_I8 1  /* v_mfma_i32_32x32x32_i8 */
 #define CGH_KERNEL_MFMA_FP4 2 /* v_mfma_scale_f32_32x32x64_f8f6f4, fp4 operands */
+#define CGH_KERNEL_VALU_POPC 3 /* VALU AND + v_bcnt_u32_b32 on 32-bit words */
 
 /* Exact H of the whole upper triangle into device memory d_out (at least
  * cgh_upper_elems(n, S) u16), stripe layout, cells with j <= i set to 0.

@@ -17,7 +17,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 
-KERNELS = [1, 2]  # CGH_KERNEL_MFMA_I8, CGH_KERNEL_MFMA_FP4
+KERNELS = [1, 2, 3]  # CGH_KERNEL_MFMA_I8, CGH_KERNEL_MFMA_FP4, CGH_KERNEL_VALU_POPC
 
 
 def gpu_upper(cgh, torch, X, tau, S, k_align=256, kernel=0):

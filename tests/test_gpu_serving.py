@@ -32,7 +32,7 @@ def expected_top(ref_rows):
     return np.array(out_d, dtype=np.uint16), np.array(out_i, dtype=np.uint32)
 
 
-@pytest.mark.parametrize("tau,slots,kernel", [(32, 4, 0), (4, 2, 1), (10**6, 3, 2)])
+@pytest.mark.parametrize("tau,slots,kernel", [(32, 4, 0), (4, 2, 1), (10**6, 3, 2), (16, 2, 3)])
 def test_server_alone_and_mixed(cgh, gpu, workloads, tau, slots, kernel):
     db_calls, q_calls, ref = make_sets(workloads)
     db = cgh.Database(db_calls, tau)
