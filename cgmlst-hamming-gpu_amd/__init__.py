@@ -25,9 +25,9 @@ CLI_PATH = os.path.join(HERE, "bin", "cgmlst-hamming-gpu")
 # Symbols the header declares, in declaration order.
 EXPORTS = (
     "tch_device_count", "tch_free_vram", "tch_run",
-    "cgh_dense_ranks", "cgh_encode", "cgh_rare_csr", "cgh_encoding_free",
+    "cgh_dense_ranks", "cgh_encode", "cgh_encode_meta", "cgh_column_table", "cgh_rare_csr", "cgh_encoding_free",
     "cgh_allele_counts", "cgh_choose_tau",
-    "cgh_dataset_upload", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
+    "cgh_dataset_upload", "cgh_dataset_build", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
     "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream",
     "cgh_batch_create", "cgh_batch_out_elems", "cgh_batch_out_offset", "cgh_batch_tile", "cgh_batch_run",
     "cgh_batch_free",
@@ -78,6 +78,9 @@ def load():
         "tch_run": (I, [I, I, S, P, S, P, S, P, P, S, I, TCH_CB, P, C.POINTER(C.c_double), C.POINTER(I), C.c_char_p, I]),
         "cgh_dense_ranks": (I, [P, I, I, P, C.c_char_p, I]),
         "cgh_encode": (I, [P, I, I, I, I, C.POINTER(Encoding), C.c_char_p, I]),
+        "cgh_encode_meta": (I, [P, I, I, I, I, C.POINTER(Encoding), C.c_char_p, I]),
+        "cgh_column_table": (S, [P, I, I, I, P, P, S]),
+        "cgh_dataset_build": (I, [I, P, I, I, I, C.POINTER(P), C.c_char_p, I]),
         "cgh_rare_csr": (I, [C.POINTER(Encoding), P, P, C.c_char_p, I]),
         "cgh_encoding_free": (None, [C.POINTER(Encoding)]),
         "cgh_allele_counts": (S, [P, I, I, P, S]),
@@ -253,11 +256,24 @@ def stripe_offset(n: int, S: int, s: int) -> int:
 class Dataset:
     """Device-resident operands (cgh_dataset_upload)."""
 
-    def __init__(self, enc: Encoded, device: int = 0):
-        self.n = enc.enc.n
+    def __init__(self, enc: Encoded = None, device: int = 0):
         self.handle = C.c_void_p()
+        if enc is None:
+            return
+        self.n = enc.enc.n
         err = _err()
         _check(load().cgh_dataset_upload(device, C.byref(enc.enc), C.byref(self.handle), err, 512), err)
+
+    @classmethod
+    def build(cls, X: np.ndarray, tau: int, device: int = 0):
+        """cgh_dataset_build: O/P built on the device from dense ranks X."""
+        X = np.ascontiguousarray(X, dtype=np.uint16)
+        ds = cls()
+        ds.n = X.shape[0]
+        err = _err()
+        _check(load().cgh_dataset_build(device, _ptr(X), X.shape[0], X.shape[1], int(tau), C.byref(ds.handle),
+                                        err, 512), err)
+        return ds
 
     def close(self):
         if _lib is not None and self.handle:

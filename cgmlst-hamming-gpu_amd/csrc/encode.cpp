@@ -108,8 +108,41 @@ extern "C" void cgh_encoding_free(cgh_encoding* enc) {
     memset(enc, 0, sizeof *enc);
 }
 
+namespace {
+int encode_impl(const uint16_t* X, int n, int L, int tau, int k_align, bool bits, cgh_encoding* enc, char* err,
+                int errlen);
+}
+
 extern "C" int cgh_encode(const uint16_t* X, int n, int L, int tau, int k_align, cgh_encoding* enc, char* err,
                           int errlen) {
+    return encode_impl(X, n, L, tau, k_align, true, enc, err, errlen);
+}
+
+extern "C" int cgh_encode_meta(const uint16_t* X, int n, int L, int tau, int k_align, cgh_encoding* enc, char* err,
+                               int errlen) {
+    return encode_impl(X, n, L, tau, k_align, false, enc, err, errlen);
+}
+
+extern "C" size_t cgh_column_table(const uint16_t* X, int n, int L, int tau, uint32_t* colstart, uint32_t* colmap,
+                                   size_t cap) {
+    std::vector<std::vector<uint32_t>> cnt;
+    locus_counts(X, n, L, cnt);
+    size_t need = 0;
+    for (int k = 0; k < L; k++) need += cnt[k].size();
+    if (!colstart || !colmap || cap < need) return need;
+    size_t o = 0, c = 0;
+    for (int k = 0; k < L; k++) {
+        colstart[k] = (uint32_t)o;
+        for (size_t a = 0; a < cnt[k].size(); a++)
+            colmap[o++] = a && cnt[k][a] >= (uint32_t)tau ? (uint32_t)c++ : 0xffffffffu;
+    }
+    colstart[L] = (uint32_t)o;
+    return need;
+}
+
+namespace {
+int encode_impl(const uint16_t* X, int n, int L, int tau, int k_align, bool bits, cgh_encoding* enc, char* err,
+                int errlen) {
     memset(enc, 0, sizeof *enc);
     if (n < 0 || L < 0 || tau < 1 || (k_align != 256 && k_align != 512 && k_align != 1024)) {
         set_err(err, errlen, "cgh_encode: bad arguments (n=%d L=%d tau=%d k_align=%d)", n, L, tau, k_align);
@@ -135,16 +168,18 @@ extern "C" int cgh_encode(const uint16_t* X, int n, int L, int tau, int k_align,
     enc->n_pad = up((size_t)std::max(n, 1), 256);
     enc->kw1 = up(std::max<size_t>(D, 1), (size_t)k_align) / 32;
     enc->kw2 = up(std::max<size_t>((size_t)L, 1), (size_t)k_align) / 32;
-    enc->O = (uint32_t*)calloc(enc->n_pad * enc->kw1, 4);
-    enc->P = (uint32_t*)calloc(enc->n_pad * enc->kw2, 4);
-    if (!enc->O || !enc->P) {
-        cgh_encoding_free(enc);
-        set_err(err, errlen, "out of host memory for the bit matrices");
-        return 1;
+    if (bits) {
+        enc->O = (uint32_t*)calloc(enc->n_pad * enc->kw1, 4);
+        enc->P = (uint32_t*)calloc(enc->n_pad * enc->kw2, 4);
+        if (!enc->O || !enc->P) {
+            cgh_encoding_free(enc);
+            set_err(err, errlen, "out of host memory for the bit matrices");
+            return 1;
+        }
     }
     // bit pack (main.rs:386-403), LSB-first
 #pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; i++) {
+    for (int i = 0; i < (bits ? n : 0); i++) {
         uint32_t* orow = enc->O + (size_t)i * enc->kw1;
         uint32_t* prow = enc->P + (size_t)i * enc->kw2;
         for (int k = 0; k < L; k++) {
@@ -252,6 +287,7 @@ extern "C" int cgh_encode(const uint16_t* X, int n, int L, int tau, int k_align,
     }
     return 0;
 }
+}  // namespace
 
 extern "C" int cgh_rare_csr(const cgh_encoding* enc, uint32_t* start, uint32_t* js, char* err, int errlen) {
     const int n = enc->n;

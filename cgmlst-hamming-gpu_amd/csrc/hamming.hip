@@ -831,41 +831,50 @@ int pick_tile(int n) { return n >= 8192 ? 256 : 128; }
 
 // Stripes r0, r0+S, ... up to row n in one launch when S is a multiple of the
 // tile (else one launch per stripe), output in the stripe layout at out.
-// Row-major operand (rows x kw words) -> planes: plane g holds words
-// 4g..4g+3 of every row (16 bytes per row, ld rows per plane).  Planes past
-// kw / 4 and rows past `rows` read zero.  One thread per 16-byte piece, in
-// plane order, so the stores are contiguous.
+// Row-major operand rows (rows x kw words) -> rows row0 .. row0+rows-1 of
+// planes: plane g holds words 4g..4g+3 of every row (16 bytes per row, ld
+// rows per plane).  Planes past kw / 4 read zero.  One thread per 16-byte
+// piece, in plane order, so the stores are contiguous.
 __global__ void __launch_bounds__(256)
-to_planes(const uint32_t* __restrict__ src, int rows, int kw, uint4* __restrict__ dst, int ld, int planes) {
+to_planes(const uint32_t* __restrict__ src, int rows, int kw, uint4* __restrict__ dst, int ld, int planes, int row0) {
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (size_t)planes * (size_t)ld) return;
-    const int g = (int)(idx / (size_t)ld), r = (int)(idx % (size_t)ld);
+    if (idx >= (size_t)planes * (size_t)rows) return;
+    const int g = (int)(idx / (size_t)rows), r = (int)(idx % (size_t)rows);
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if (r < rows && 4 * g + 3 < kw) {
+    if (4 * g + 3 < kw) {
         const uint32_t* p = src + (size_t)r * kw + 4 * g;
         v = make_uint4(p[0], p[1], p[2], p[3]);
     }
-    dst[idx] = v;
+    dst[(size_t)g * ld + row0 + r] = v;
 }
 
-hipError_t launch_planes(const uint32_t* src, int rows, int kw, uint32_t* dst, int ld, int kwd, hipStream_t st) {
-    const size_t pieces = (size_t)(kwd / 4) * (size_t)ld;
+hipError_t launch_planes(const uint32_t* src, int rows, int kw, uint32_t* dst, int ld, int kwd, hipStream_t st,
+                         int row0 = 0) {
+    const size_t pieces = (size_t)(kwd / 4) * (size_t)rows;
     hipLaunchKernelGGL(to_planes, dim3((unsigned)((pieces + 255) / 256)), dim3(256), 0, st, src, rows, kw,
-                       reinterpret_cast<uint4*>(dst), ld, kwd / 4);
+                       reinterpret_cast<uint4*>(dst), ld, kwd / 4, row0);
     return hipGetLastError();
 }
 
 // Host row-major operand (ld x kw words) -> a new device buffer in planes of
-// kwd words (kwd >= kw, a multiple of KC).
+// kwd words (kwd >= kw, a multiple of KC), converted through a staging
+// buffer of at most 256 MiB of rows at a time.
 hipError_t upload_planes(const void* host, int ld, int kw, int kwd, uint32_t** out) {
     uint32_t* tmp = NULL;
     hipError_t e;
+    const size_t row_bytes = (size_t)kw * 4;
+    const int block = (int)std::max<size_t>(256, std::min<size_t>((size_t)ld, ((256u << 20) / row_bytes) / 256 * 256));
     *out = NULL;
-    if ((e = hipMalloc(&tmp, (size_t)ld * kw * 4))) return e;
+    if ((e = hipMalloc(&tmp, (size_t)block * row_bytes))) return e;
     if ((e = hipMalloc(out, (size_t)ld * kwd * 4))) goto done;
-    if ((e = hipMemcpy(tmp, host, (size_t)ld * kw * 4, hipMemcpyHostToDevice))) goto done;
-    if ((e = launch_planes(tmp, ld, kw, *out, ld, kwd, NULL))) goto done;
-    e = hipDeviceSynchronize();
+    for (int r0 = 0; r0 < ld; r0 += block) {
+        const int rows = std::min(block, ld - r0);
+        if ((e = hipMemcpy(tmp, (const char*)host + (size_t)r0 * row_bytes, (size_t)rows * row_bytes,
+                           hipMemcpyHostToDevice)))
+            goto done;
+        if ((e = launch_planes(tmp, rows, kw, *out, ld, kwd, NULL, r0))) goto done;
+        if ((e = hipDeviceSynchronize())) goto done;  // the staging buffer is reused
+    }
 done:
     (void)hipFree(tmp);
     if (e && *out) {
@@ -873,6 +882,23 @@ done:
         *out = NULL;
     }
     return e;
+}
+
+// Device operand build from dense ranks X (n x L u16, device): P bit k and
+// O bit colmap[colstart[k] + a] of every called cell, ORed into zeroed planes
+// of ld rows.  Workgroup b: row b / lb, loci 256 (b % lb) .. +255.
+__global__ void __launch_bounds__(256)
+build_planes(const uint16_t* __restrict__ X, int L, int lb, const uint32_t* __restrict__ colstart,
+             const uint32_t* __restrict__ colmap, uint32_t* __restrict__ O, uint32_t* __restrict__ P, int ld) {
+    const int i = (int)(blockIdx.x / (unsigned)lb);
+    const int k = (int)(blockIdx.x % (unsigned)lb) * 256 + (int)threadIdx.x;
+    if (k >= L) return;
+    const int a = X[(size_t)i * L + k];
+    if (!a) return;
+    auto word = [&](uint32_t bit) { return (((size_t)(bit >> 7) * ld + i) << 2) + ((bit >> 5) & 3); };
+    atomicOr(&P[word((uint32_t)k)], 1u << (k & 31));
+    const uint32_t c = colmap[colstart[k] + a];
+    if (c != 0xffffffffu) atomicOr(&O[word(c)], 1u << (c & 31));
 }
 
 int kc_words(size_t kw) { return (int)up(kw, (size_t)KC); }
@@ -1085,7 +1111,10 @@ extern "C" void cgh_dataset_free(cgh_dataset* ds) {
     free(ds);
 }
 
-extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_dataset** out, char* err, int errlen) {
+namespace {
+
+// A dataset's sizes and rare structures from an encoding (O/P not touched).
+int dataset_meta(int device, const cgh_encoding* enc, cgh_dataset** out, char* err, int errlen) {
     cgh_dataset* ds = (cgh_dataset*)calloc(1, sizeof(cgh_dataset));
     if (!ds) {
         set_err(err, errlen, "out of host memory");
@@ -1108,8 +1137,6 @@ extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_datas
         return 1;
     }
     HK(hipSetDevice(device));
-    HK(upload_planes(enc->O, (int)enc->n_pad, (int)enc->kw1, (int)ds->kw1, &ds->dO));
-    HK(upload_planes(enc->P, (int)enc->n_pad, (int)enc->kw2, (int)ds->kw2, &ds->dP));
     if (enc->n_groups) {
         ds->m16 = enc->n <= 65536;
         HK(upload_members(enc->gmem, enc->n_members, ds->m16, &ds->dgmem));
@@ -1122,6 +1149,75 @@ extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_datas
     *out = ds;
     return 0;
 fail:
+    cgh_dataset_free(ds);
+    return 1;
+}
+
+}  // namespace
+
+extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_dataset** out, char* err, int errlen) {
+    cgh_dataset* ds = NULL;
+    *out = NULL;
+    if (dataset_meta(device, enc, &ds, err, errlen)) return 1;
+    HK(upload_planes(enc->O, (int)enc->n_pad, (int)enc->kw1, (int)ds->kw1, &ds->dO));
+    HK(upload_planes(enc->P, (int)enc->n_pad, (int)enc->kw2, (int)ds->kw2, &ds->dP));
+    *out = ds;
+    return 0;
+fail:
+    cgh_dataset_free(ds);
+    return 1;
+}
+
+extern "C" int cgh_dataset_build(int device, const uint16_t* X, int n, int L, int tau, cgh_dataset** out,
+                                 char* err, int errlen) {
+    cgh_encoding enc;
+    cgh_dataset* ds = NULL;
+    uint16_t* dX = NULL;
+    uint32_t *dcs = NULL, *dcm = NULL;
+    std::vector<uint32_t> colstart, colmap;
+    size_t need = 0;
+    *out = NULL;
+    if (n < 1 || L < 1) {
+        set_err(err, errlen, "cgh_dataset_build: bad arguments (n=%d L=%d)", n, L);
+        return 1;
+    }
+    if (cgh_encode_meta(X, n, L, tau, 256, &enc, err, errlen)) return 1;
+    if (dataset_meta(device, &enc, &ds, err, errlen)) {
+        cgh_encoding_free(&enc);
+        return 1;
+    }
+    need = cgh_column_table(X, n, L, tau, NULL, NULL, 0);
+    colstart.resize((size_t)L + 1);
+    colmap.resize(need);
+    cgh_column_table(X, n, L, tau, colstart.data(), colmap.data(), need);
+    HK(hipMalloc(&ds->dO, ds->n_pad * ds->kw1 * 4));
+    HK(hipMalloc(&ds->dP, ds->n_pad * ds->kw2 * 4));
+    HK(hipMemset(ds->dO, 0, ds->n_pad * ds->kw1 * 4));
+    HK(hipMemset(ds->dP, 0, ds->n_pad * ds->kw2 * 4));
+    HK(hipMalloc(&dX, (size_t)n * L * 2));
+    HK(hipMalloc(&dcs, colstart.size() * 4));
+    HK(hipMalloc(&dcm, std::max<size_t>(need, 1) * 4));
+    HK(hipMemcpy(dX, X, (size_t)n * L * 2, hipMemcpyHostToDevice));
+    HK(hipMemcpy(dcs, colstart.data(), colstart.size() * 4, hipMemcpyHostToDevice));
+    HK(hipMemcpy(dcm, colmap.data(), need * 4, hipMemcpyHostToDevice));
+    {
+        const int lb = (L + 255) / 256;
+        hipLaunchKernelGGL(build_planes, dim3((unsigned)((size_t)n * lb)), dim3(256), 0, 0, dX, L, lb, dcs, dcm,
+                           ds->dO, ds->dP, (int)ds->n_pad);
+        HK(hipGetLastError());
+        HK(hipDeviceSynchronize());
+    }
+    (void)hipFree(dX);
+    (void)hipFree(dcs);
+    (void)hipFree(dcm);
+    cgh_encoding_free(&enc);
+    *out = ds;
+    return 0;
+fail:
+    if (dX) (void)hipFree(dX);
+    if (dcs) (void)hipFree(dcs);
+    if (dcm) (void)hipFree(dcm);
+    cgh_encoding_free(&enc);
     cgh_dataset_free(ds);
     return 1;
 }

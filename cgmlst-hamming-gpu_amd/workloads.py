@@ -77,14 +77,16 @@ def batch_shapes(count=64, seed=4, n_range=(500, 20000), L_range=(800, 3000)):
 
 
 def founders(n=100000, L=5000, seed=3, nf=50, mut=0.01, miss=0.01):
+    """Dense-frequent founder model: every locus has nf founder alleles (ids
+    1..nf); each profile copies one of them per locus, a fraction `mut` of the
+    calls mutate to a fresh id never seen elsewhere, a fraction `miss` is
+    missing.  Generated in row blocks (n x L u64)."""
     rng = np.random.default_rng(seed)
-    F = rng.integers(1, nf + 1, size=(nf, L), dtype=np.int64)
-    pick = rng.integers(0, nf, size=n)
     out = np.empty((n, L), dtype=np.uint64)
     next_id = nf + 1
     for i0 in range(0, n, 4096):
         i1 = min(n, i0 + 4096)
-        blk = F[pick[i0:i1]].astype(np.uint64)
+        blk = rng.integers(1, nf + 1, size=(i1 - i0, L), dtype=np.uint64)
         m = rng.random(blk.shape) < mut
         cnt = int(m.sum())
         blk[m] = np.arange(next_id, next_id + cnt, dtype=np.uint64)

@@ -98,6 +98,19 @@ typedef struct cgh_encoding {
 int cgh_encode(const uint16_t* X, int n, int L, int tau, int k_align, cgh_encoding* enc,
                char* err, int errlen);
 
+/* cgh_encode without the O/P bit matrices (O = P = NULL; every size field
+ * and the rare groups as cgh_encode gives them): the metadata a device-side
+ * operand build (cgh_dataset_build) needs. */
+int cgh_encode_meta(const uint16_t* X, int n, int L, int tau, int k_align, cgh_encoding* enc,
+                    char* err, int errlen);
+
+/* Column table of cgh_encode's one-hot map: colmap[colstart[k] + a] is the
+ * O column of rank a at locus k (a = 0..max rank), or 0xFFFFFFFF for missing
+ * and for alleles with count < tau.  colstart has L + 1 entries.  Returns the
+ * number of colmap entries needed; fills the table only when cap is enough. */
+size_t cgh_column_table(const uint16_t* X, int n, int L, int tau, uint32_t* colstart, uint32_t* colmap,
+                        size_t cap);
+
 /* Expand the rare groups of `enc` into the reference's CSR of partners j > i
  * per row (start[n+1], js[n_rare]; caller-allocated).  Fails with "too many
  * rare pairs: raise --tau" when n_rare >= 2^32.
@@ -130,6 +143,13 @@ typedef struct cgh_dataset cgh_dataset;
 
 /* Upload O, P and the rare groups of `enc` to `device`. */
 int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_dataset** out, char* err, int errlen);
+
+/* The same dataset from dense ranks X (n x L u16) at threshold tau, with the
+ * O/P bits built on the device from X and the column table (no host bit
+ * matrices: at 100,000 x 5,000 and tau = 1, O alone is 66 GB).  The rare
+ * groups are encoded on the host as cgh_encode does. */
+int cgh_dataset_build(int device, const uint16_t* X, int n, int L, int tau, cgh_dataset** out, char* err,
+                      int errlen);
 void cgh_dataset_free(cgh_dataset* ds);
 
 /* u16 elements of the upper-triangle stripe layout for n profiles and stripe
