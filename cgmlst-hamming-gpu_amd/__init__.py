@@ -27,7 +27,7 @@ EXPORTS = (
     "tch_device_count", "tch_free_vram", "tch_run",
     "cgh_dense_ranks", "cgh_encode", "cgh_encode_meta", "cgh_column_table", "cgh_rare_csr", "cgh_encoding_free",
     "cgh_allele_counts", "cgh_choose_tau",
-    "cgh_dataset_upload", "cgh_dataset_build", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
+    "cgh_dataset_upload", "cgh_dataset_build", "cgh_dataset_encode", "cgh_dataset_info", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
     "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream",
     "cgh_batch_create", "cgh_batch_out_elems", "cgh_batch_out_offset", "cgh_batch_tile", "cgh_batch_run",
     "cgh_batch_free",
@@ -81,6 +81,8 @@ def load():
         "cgh_encode_meta": (I, [P, I, I, I, I, C.POINTER(Encoding), C.c_char_p, I]),
         "cgh_column_table": (S, [P, I, I, I, P, P, S]),
         "cgh_dataset_build": (I, [I, P, I, I, I, C.POINTER(P), C.c_char_p, I]),
+        "cgh_dataset_encode": (I, [I, P, I, I, I, S, P, P, C.POINTER(P), C.POINTER(I), P, C.c_char_p, I]),
+        "cgh_dataset_info": (None, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(S), C.POINTER(U64)]),
         "cgh_rare_csr": (I, [C.POINTER(Encoding), P, P, C.c_char_p, I]),
         "cgh_encoding_free": (None, [C.POINTER(Encoding)]),
         "cgh_allele_counts": (S, [P, I, I, P, S]),
@@ -274,6 +276,28 @@ class Dataset:
         _check(load().cgh_dataset_build(device, _ptr(X), X.shape[0], X.shape[1], int(tau), C.byref(ds.handle),
                                         err, 512), err)
         return ds
+
+    @classmethod
+    def encode(cls, calls: np.ndarray, tau: int = 0, out_bytes: int = 0, device: int = 0, want_ranks: bool = False):
+        """cgh_dataset_encode: u64 ids encoded on the device.  -> (dataset,
+        tau used, dense ranks or None)"""
+        calls = np.ascontiguousarray(calls, dtype=np.uint64)
+        n, L = calls.shape
+        X = np.zeros((n, L), dtype=np.uint16) if want_ranks else None
+        ds = cls()
+        ds.n = n
+        t = C.c_int()
+        err = _err()
+        _check(load().cgh_dataset_encode(device, _ptr(calls), n, L, int(tau), int(out_bytes), None, None,
+                                         C.byref(ds.handle), C.byref(t), _ptr(X) if X is not None else None, err,
+                                         512), err)
+        return ds, t.value, X
+
+    def info(self):
+        """-> dict(n, L, tau, D, n_rare)"""
+        v = [C.c_int(), C.c_int(), C.c_int(), C.c_size_t(), C.c_uint64()]
+        load().cgh_dataset_info(self.handle, *[C.byref(x) for x in v])
+        return dict(zip(("n", "L", "tau", "D", "n_rare"), (x.value for x in v)))
 
     def close(self):
         if _lib is not None and self.handle:

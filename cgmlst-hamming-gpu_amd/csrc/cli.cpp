@@ -547,43 +547,36 @@ int main(int argc, char** argv) {
     }
     if (l > 65535) die("more than 65535 loci (u16 distances)");
 
-    // ---- encode (main.rs:353-445) ----
+    // ---- encode (main.rs:353-445), on the device ----
     t = now();
     char err[512] = {0};
-    std::vector<uint16_t> X(n * l);
-    if (cgh_dense_ranks(calls.data(), (int)n, (int)l, X.data(), err, sizeof err)) die(err);
-    std::vector<uint64_t>().swap(calls);
     Upper up;
     up.rows.resize(n);
+    cgh_dataset* ds = NULL;
+    const size_t stripe = std::max<size_t>(args.stripe, 8) / 8 * 8;
     if (n >= 2 && l > 0) {
-        const size_t stripe = std::max<size_t>(args.stripe, 8) / 8 * 8;
-        size_t tau = args.tau;
-        if (tau == 0) {
-            std::vector<uint32_t> cnt(cgh_allele_counts(X.data(), (int)n, (int)l, NULL, 0));
-            cgh_allele_counts(X.data(), (int)n, (int)l, cnt.data(), cnt.size());
-            const size_t freeb = tch_free_vram((int)args.device);
-            const size_t n_pad = (n + 255) / 256 * 256;
-            const size_t out_bytes = 2 * ((stripe * n_pad + 7) / 8 * 8) * 2;  // two stripe buffers
-            tau = (size_t)cgh_choose_tau(cnt.data(), cnt.size(), (int)n, (int)l, out_bytes, freeb, log_line,
-                                         &args.silent);
-            if (tau == 0) {
-                snprintf(msg, sizeof msg, "no tau fits in %.2f GB of free VRAM", (double)freeb / 1e9);
-                die(msg);
-            }
-        }
-        const int tau_i = (int)std::min<size_t>(tau, (size_t)n + 1);  // counts never exceed n
-        cgh_encoding enc;
-        if (cgh_encode(X.data(), (int)n, (int)l, tau_i, 256, &enc, err, sizeof err)) die(err);
-        snprintf(msg, sizeof msg, "Encoded in %.2fs: tau=%zu, one-hot cols=%zu, rare equal pairs=%llu", now() - t, tau,
-                 enc.D, (unsigned long long)enc.n_rare);
+        const size_t n_pad = (n + 255) / 256 * 256;
+        const size_t out_bytes = 2 * ((stripe * n_pad + 7) / 8 * 8) * 2;  // two stripe buffers
+        const int tau_i = (int)std::min<size_t>(args.tau, (size_t)n + 1);  // counts never exceed n; 0 = auto
+        int tau_used = 0;
+        if (cgh_dataset_encode((int)args.device, calls.data(), (int)n, (int)l, tau_i, out_bytes, log_line,
+                               &args.silent, &ds, &tau_used, NULL, err, sizeof err))
+            die(strncmp(err, "hip", 3) ? std::string(err) : std::string("GPU: ") + err);
+        size_t D = 0;
+        uint64_t n_rare = 0;
+        cgh_dataset_info(ds, NULL, NULL, NULL, &D, &n_rare);
+        snprintf(msg, sizeof msg, "Encoded in %.2fs: tau=%d, one-hot cols=%zu, rare equal pairs=%llu", now() - t,
+                 tau_used, D, (unsigned long long)n_rare);
         log(msg);
-
+    } else {  // nothing to compute; the rank limit still applies
+        std::vector<uint16_t> X(n * l);
+        if (cgh_dense_ranks(calls.data(), (int)n, (int)l, X.data(), err, sizeof err)) die(err);
+    }
+    std::vector<uint64_t>().swap(calls);
+    if (ds) {
         // ---- GPU (main.rs:447-465) ----
         t = now();
         double gpu_secs = 0;
-        cgh_dataset* ds = NULL;
-        if (cgh_dataset_upload((int)args.device, &enc, &ds, err, sizeof err)) die(std::string("GPU: ") + err);
-        cgh_encoding_free(&enc);
         const int rc = cgh_dataset_stream(ds, (int)stripe, on_stripe, &up, &gpu_secs, err, sizeof err);
         cgh_dataset_free(ds);
         if (rc) die(std::string("GPU: ") + err);

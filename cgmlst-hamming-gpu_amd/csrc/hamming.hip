@@ -46,6 +46,7 @@
 #include <vector>
 
 #include "cgmlst_hamming.h"
+#include "internal.h"
 
 namespace {
 
@@ -1088,7 +1089,8 @@ fail:
 
 struct cgh_dataset {
     int device;
-    int n, L;
+    int n, L, tau;
+    size_t D;                 // one-hot columns
     size_t n_pad, kw1, kw2;   // kw1, kw2: words per row on the device (multiples of KC)
     uint32_t *dO, *dP;        // planes of n_pad rows
     size_t n_groups, n_members;
@@ -1124,6 +1126,8 @@ int dataset_meta(int device, const cgh_encoding* enc, cgh_dataset** out, char* e
     ds->device = device;
     ds->n = enc->n;
     ds->L = enc->L;
+    ds->tau = enc->tau;
+    ds->D = enc->D;
     ds->n_pad = enc->n_pad;
     ds->kw1 = (size_t)kc_words(enc->kw1);
     ds->kw2 = (size_t)kc_words(enc->kw2);
@@ -1220,6 +1224,52 @@ fail:
     cgh_encoding_free(&enc);
     cgh_dataset_free(ds);
     return 1;
+}
+
+namespace cghi {
+
+cgh_dataset* dataset_adopt(const DatasetParts& p) {
+    cgh_dataset* ds = (cgh_dataset*)calloc(1, sizeof(cgh_dataset));
+    if (!ds) return NULL;
+    ds->device = p.device;
+    ds->n = p.n;
+    ds->L = p.L;
+    ds->tau = p.tau;
+    ds->D = p.D;
+    ds->n_pad = p.n_pad;
+    ds->kw1 = p.kw1;
+    ds->kw2 = p.kw2;
+    ds->dO = p.dO;
+    ds->dP = p.dP;
+    ds->n_groups = p.n_groups;
+    ds->n_members = p.n_members;
+    ds->n_rare = p.n_rare;
+    ds->n_rentries = p.n_rentries;
+    ds->dgmem = p.dgmem;
+    ds->m16 = p.m16;
+    ds->drstart = p.drstart;
+    ds->drentry = p.drentry;
+    return ds;
+}
+
+int device_words(size_t kw) { return kc_words(kw); }
+
+hipError_t launch_build_planes(const uint16_t* dX, int n, int L, const uint32_t* dcolstart, const uint32_t* dcolmap,
+                               uint32_t* dO, uint32_t* dP, int ld, hipStream_t st) {
+    const int lb = (L + 255) / 256;
+    hipLaunchKernelGGL(build_planes, dim3((unsigned)((size_t)n * lb)), dim3(256), 0, st, dX, L, lb, dcolstart,
+                       dcolmap, dO, dP, ld);
+    return hipGetLastError();
+}
+
+}  // namespace cghi
+
+extern "C" void cgh_dataset_info(const cgh_dataset* ds, int* n, int* L, int* tau, size_t* D, uint64_t* n_rare) {
+    if (n) *n = ds->n;
+    if (L) *L = ds->L;
+    if (tau) *tau = ds->tau;
+    if (D) *D = ds->D;
+    if (n_rare) *n_rare = ds->n_rare;
 }
 
 extern "C" size_t cgh_stripe_offset(int n, int S, int s) { return stripe_off((size_t)n, (size_t)S, (size_t)s); }

@@ -1,0 +1,39 @@
+// internal.h -- what the device encoder (gpu_encode.hip) needs from the
+// dataset code in hamming.hip.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cgmlst_hamming.h"
+
+namespace cghi {
+
+// Device buffers and sizes of a dataset; cgh_dataset takes ownership.
+struct DatasetParts {
+    int device = 0, n = 0, L = 0, tau = 0;
+    size_t D = 0;                   // one-hot columns
+    size_t n_pad = 0;
+    size_t kw1 = 0, kw2 = 0;        // device words per row of O and P (multiples of the product slab)
+    uint32_t* dO = nullptr;         // planes of n_pad rows
+    uint32_t* dP = nullptr;
+    size_t n_groups = 0, n_members = 0, n_rentries = 0;
+    uint64_t n_rare = 0;
+    void* dgmem = nullptr;          // members, u16 when m16
+    int m16 = 0;
+    uint32_t* drstart = nullptr;    // n + 1
+    uint64_t* drentry = nullptr;
+};
+
+cgh_dataset* dataset_adopt(const DatasetParts& p);
+
+// Device words per row for host words kw (rounded up to the product slab).
+int device_words(size_t kw);
+
+// O/P planes (zeroed by the caller) from dense ranks dX (n x L u16, device)
+// and the column table (colstart[L + 1], colmap), rows per plane ld.
+hipError_t launch_build_planes(const uint16_t* dX, int n, int L, const uint32_t* dcolstart, const uint32_t* dcolmap,
+                               uint32_t* dO, uint32_t* dP, int ld, hipStream_t st);
+
+}  // namespace cghi

@@ -150,6 +150,23 @@ int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_dataset** out, c
  * groups are encoded on the host as cgh_encode does. */
 int cgh_dataset_build(int device, const uint16_t* X, int n, int L, int tau, cgh_dataset** out, char* err,
                       int errlen);
+
+/* The encoder on the device: u64 allele ids (n x L, 0 = missing, n * L <
+ * 2^31) straight to a dataset.  Per-locus dense ranks (main.rs:134-165, the
+ * same error past 65,535 distinct alleles), allele counts, the O/P planes,
+ * the rare groups (members ascending) and their row index are computed on
+ * the device; only per-allele counts travel to the host, to choose tau
+ * (tau = 0: cgh_choose_tau with out_bytes and the device's free memory) and
+ * to lay out the column and group tables.  tau_used (may be NULL) receives
+ * the threshold; X (may be NULL, n x L u16) receives the dense ranks; log
+ * (may be NULL) receives cgh_choose_tau's table. */
+int cgh_dataset_encode(int device, const uint64_t* calls, int n, int L, int tau, size_t out_bytes,
+                       void (*log)(void* ctx, const char* line), void* log_ctx, cgh_dataset** out, int* tau_used,
+                       uint16_t* X, char* err, int errlen);
+
+/* Sizes of a dataset: profiles, loci, tau, one-hot columns, rare equal
+ * pairs (any pointer may be NULL). */
+void cgh_dataset_info(const cgh_dataset* ds, int* n, int* L, int* tau, size_t* D, uint64_t* n_rare);
 void cgh_dataset_free(cgh_dataset* ds);
 
 /* u16 elements of the upper-triangle stripe layout for n profiles and stripe
