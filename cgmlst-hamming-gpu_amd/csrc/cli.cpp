@@ -350,31 +350,11 @@ struct Upper {
     std::vector<std::vector<uint16_t>> rows;  // row i: H[i][i+1..n-1]
 };
 
-void on_stripe(void* ctx, int r0, int rows, size_t w, const uint16_t* stripe) {
-    Upper* u = (Upper*)ctx;
-#pragma omp parallel for schedule(static)
-    for (int r = 0; r < rows; r++)
-        u->rows[(size_t)r0 + r].assign(stripe + (size_t)r * w + r + 1, stripe + (size_t)(r + 1) * w);
-}
-
 inline uint16_t at(const Upper& u, size_t i, size_t j) {
     if (i == j) return 0;
     if (j > i) return u.rows[i][j - i - 1];
     return u.rows[j][i - j - 1];
 }
-
-struct Sink {
-    FILE* f = NULL;
-    gzFile z = NULL;
-    void write(const char* p, size_t len) {
-        if (!len) return;
-        if (z) {
-            if (gzwrite(z, p, (unsigned)len) != (int)len) die("writing output: gzip error");
-        } else if (fwrite(p, 1, len, f) != len) {
-            die("writing output: " + io_error(errno));
-        }
-    }
-};
 
 char* fmt_u(char* p, uint32_t v) {
     char tmp[12];
@@ -387,42 +367,144 @@ char* fmt_u(char* p, uint32_t v) {
     return p;
 }
 
-// main.rs:225-265: header, then rows formatted in parallel chunks of 1024.
-void write_matrix(Sink& out, const std::vector<std::string>& ids, const Upper& up, const std::string& osep,
-                  const std::string& index_name, Layout layout, int32_t cap) {
-    const size_t n = ids.size();
-    std::string head = index_name;
-    for (auto& id : ids) {
-        head += osep;
-        head += id;
+// The output text, written row by row in order (main.rs:225-265).  Row i
+// needs H[i][j] for every j: the upper part of rows j <= i, so a row can be
+// written as soon as its own stripe has arrived.  Rows are formatted in
+// parallel in groups of GROUP; with gzip each group becomes one gzip member
+// (level 3), compressed in parallel too: concatenated members are one valid
+// gzip stream.  The file is opened at the first write.
+struct Writer {
+    static constexpr size_t GROUP = 64;
+    const std::vector<std::string>* ids = NULL;
+    const Upper* up = NULL;
+    std::string osep, index_name, path;
+    Layout layout = Layout::Full;
+    int32_t cap = 0;
+    bool gz = false, to_file = false;
+    FILE* f = NULL;
+    size_t next = 0;  // rows written
+
+    void put(const char* p, size_t len) {
+        if (len && fwrite(p, 1, len, f) != len) die("writing output: " + io_error(errno));
     }
-    head += "\n";
-    out.write(head.data(), head.size());
-    const size_t CHUNK = 1024;
-    std::vector<std::string> bufs(CHUNK);
-    for (size_t s = 0; s < n; s += CHUNK) {
-        const size_t e = std::min(n, s + CHUNK);
-#pragma omp parallel for schedule(dynamic, 1)
-        for (long ii = (long)s; ii < (long)e; ii++) {
-            const size_t i = (size_t)ii;
-            std::string& b = bufs[i - s];
-            b.resize(ids[i].size() + n * (osep.size() + 11) + 1);
-            char* p = &b[0];
-            memcpy(p, ids[i].data(), ids[i].size());
-            p += ids[i].size();
-            for (size_t j = 0; j < n; j++) {
-                memcpy(p, osep.data(), osep.size());
-                p += osep.size();
-                bool keep = layout == Layout::Full || (layout == Layout::LowerTri ? j <= i : j >= i);
-                int32_t v = keep ? (int32_t)at(up, i, j) : 0;
-                if (cap > 0 && v > cap) v = cap;
-                p = fmt_u(p, (uint32_t)v);
-            }
-            *p++ = '\n';
-            b.resize((size_t)(p - &b[0]));
+
+    void open() {
+        if (f) return;
+        if (to_file) {
+            f = fopen(path.c_str(), "wb");
+            if (!f) die("creating " + path + ": " + io_error(errno));
+        } else {
+            f = stdout;
         }
-        for (size_t i = s; i < e; i++) out.write(bufs[i - s].data(), bufs[i - s].size());
+        setvbuf(f, NULL, _IOFBF, 16 << 20);
+        std::string head = index_name;
+        for (auto& id : *ids) {
+            head += osep;
+            head += id;
+        }
+        head += "\n";
+        emit({head});
     }
+
+    // Text pieces in order -> the file (one gzip member for all of them).
+    void emit(const std::vector<std::string>& parts) {
+        if (!gz) {
+            for (auto& t : parts) put(t.data(), t.size());
+            return;
+        }
+        std::string z = deflate_member(parts);
+        put(z.data(), z.size());
+    }
+
+    static std::string deflate_member(const std::vector<std::string>& parts) {
+        z_stream zs;
+        memset(&zs, 0, sizeof zs);
+        if (deflateInit2(&zs, 3, Z_DEFLATED, 15 + 16, 8, Z_DEFAULT_STRATEGY) != Z_OK) die("gzip: deflateInit2");
+        size_t in = 0;
+        for (auto& t : parts) in += t.size();
+        std::string out(deflateBound(&zs, (uLong)in) + parts.size() * 16 + 64, '\0');
+        zs.next_out = (Bytef*)&out[0];
+        zs.avail_out = (uInt)out.size();
+        for (size_t k = 0; k < parts.size(); k++) {
+            zs.next_in = (Bytef*)parts[k].data();
+            zs.avail_in = (uInt)parts[k].size();
+            const int flush = k + 1 == parts.size() ? Z_FINISH : Z_NO_FLUSH;
+            const int rc = deflate(&zs, flush);
+            if (rc == Z_STREAM_ERROR || (flush == Z_FINISH && rc != Z_STREAM_END)) die("writing output: gzip error");
+        }
+        if (parts.empty()) {
+            if (deflate(&zs, Z_FINISH) != Z_STREAM_END) die("writing output: gzip error");
+        }
+        out.resize(zs.total_out);
+        deflateEnd(&zs);
+        return out;
+    }
+
+    void format_row(size_t i, std::string& b) const {
+        const size_t n = ids->size();
+        b.resize((*ids)[i].size() + n * (osep.size() + 11) + 1);
+        char* p = &b[0];
+        memcpy(p, (*ids)[i].data(), (*ids)[i].size());
+        p += (*ids)[i].size();
+        for (size_t j = 0; j < n; j++) {
+            memcpy(p, osep.data(), osep.size());
+            p += osep.size();
+            const bool keep = layout == Layout::Full || (layout == Layout::LowerTri ? j <= i : j >= i);
+            int32_t v = keep ? (int32_t)at(*up, i, j) : 0;
+            if (cap > 0 && v > cap) v = cap;
+            p = fmt_u(p, (uint32_t)v);
+        }
+        *p++ = '\n';
+        b.resize((size_t)(p - &b[0]));
+    }
+
+    // Rows next .. i1-1 (all their stripes have arrived).
+    void rows_until(size_t i1) {
+        open();
+        while (next < i1) {
+            const size_t i0 = next, e = std::min(i1, i0 + 16 * GROUP);
+            const size_t groups = (e - i0 + GROUP - 1) / GROUP;
+            std::vector<std::vector<std::string>> text(groups);
+            std::vector<std::string> packed(groups);
+#pragma omp parallel for schedule(dynamic, 1)
+            for (long g = 0; g < (long)groups; g++) {
+                const size_t a = i0 + (size_t)g * GROUP, z = std::min(e, a + GROUP);
+                text[g].resize(z - a);
+                for (size_t i = a; i < z; i++) format_row(i, text[g][i - a]);
+                if (gz) {
+                    packed[g] = deflate_member(text[g]);
+                    std::vector<std::string>().swap(text[g]);
+                }
+            }
+            for (size_t g = 0; g < groups; g++) {
+                if (gz) put(packed[g].data(), packed[g].size());
+                else
+                    for (auto& t : text[g]) put(t.data(), t.size());
+            }
+            next = e;
+        }
+    }
+
+    void finish() {
+        open();
+        if (fflush(f) != 0) die("writing output: " + io_error(errno));
+        if (f != stdout) fclose(f);
+    }
+};
+
+struct StripeCtx {
+    Upper* up;
+    Writer* w;
+};
+
+// Stripe callback (tch_cb): keep the strict upper rows, then write every row
+// of the stripe while the device computes the next one.
+void on_stripe(void* ctx, int r0, int rows, size_t w, const uint16_t* stripe) {
+    StripeCtx* c = (StripeCtx*)ctx;
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < rows; r++)
+        c->up->rows[(size_t)r0 + r].assign(stripe + (size_t)r * w + r + 1, stripe + (size_t)(r + 1) * w);
+    c->w->rows_until((size_t)r0 + rows);
 }
 
 void log_line(void* ctx, const char* line) {
@@ -547,11 +629,24 @@ int main(int argc, char** argv) {
     }
     if (l > 65535) die("more than 65535 loci (u16 distances)");
 
+    // ---- output (main.rs:472-490): written as the stripes arrive ----
+    Writer wr;
+    wr.ids = &ids;
+    wr.osep = args.output_sep;
+    wr.index_name = args.index_name;
+    wr.layout = args.layout;
+    wr.cap = args.max_dist;
+    wr.to_file = args.have_output;
+    wr.path = args.output;
+    wr.gz = args.gzip || (args.have_output && args.output.size() >= 3 &&
+                          args.output.compare(args.output.size() - 3, 3, ".gz") == 0);
+
     // ---- encode (main.rs:353-445), on the device ----
     t = now();
     char err[512] = {0};
     Upper up;
     up.rows.resize(n);
+    wr.up = &up;
     cgh_dataset* ds = NULL;
     const size_t stripe = std::max<size_t>(args.stripe, 8) / 8 * 8;
     if (n >= 2 && l > 0) {
@@ -577,7 +672,8 @@ int main(int argc, char** argv) {
         // ---- GPU (main.rs:447-465) ----
         t = now();
         double gpu_secs = 0;
-        const int rc = cgh_dataset_stream(ds, (int)stripe, on_stripe, &up, &gpu_secs, err, sizeof err);
+        StripeCtx sc = {&up, &wr};
+        const int rc = cgh_dataset_stream(ds, (int)stripe, on_stripe, &sc, &gpu_secs, err, sizeof err);
         cgh_dataset_free(ds);
         if (rc) die(std::string("GPU: ") + err);
         snprintf(msg, sizeof msg, "Computed %zu x %zu matrix on MFMA in %.2fs (GPU pipeline %.2fs, rare pairs on GPU)",
@@ -587,27 +683,10 @@ int main(int argc, char** argv) {
         for (size_t i = 0; i < n; i++) up.rows[i].assign(n - 1 - i, 0);
     }
 
-    // ---- write (main.rs:472-490) ----
+    // ---- the rows not written yet (all of them when nothing ran on the device) ----
     t = now();
-    const bool gz = args.gzip || (args.have_output && args.output.size() >= 3 &&
-                                  args.output.compare(args.output.size() - 3, 3, ".gz") == 0);
-    Sink sink;
-    if (args.have_output) {
-        sink.f = fopen(args.output.c_str(), "wb");
-        if (!sink.f) die("creating " + args.output + ": " + io_error(errno));
-    } else {
-        sink.f = stdout;
-    }
-    if (gz) {
-        sink.z = gzdopen(dup(fileno(sink.f)), "wb3");
-        if (!sink.z) die("creating gzip stream");
-    } else {
-        setvbuf(sink.f, NULL, _IOFBF, 16 << 20);
-    }
-    write_matrix(sink, ids, up, args.output_sep, args.index_name, args.layout, args.max_dist);
-    if (sink.z && gzclose(sink.z) != Z_OK) die("writing output: gzip error");
-    if (fflush(sink.f) != 0) die("writing output: " + io_error(errno));
-    if (sink.f != stdout) fclose(sink.f);
+    wr.rows_until(n);
+    wr.finish();
     snprintf(msg, sizeof msg, "Wrote output in %.2fs. Total %.2fs", now() - t, now() - t_start);
     log(msg);
     return 0;

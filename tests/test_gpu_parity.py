@@ -54,6 +54,35 @@ def test_check_matrix_cli(cgh, gpu, case):
             assert np.array_equal(got, want), (case, args)
 
 
+def test_cli_gzip_and_file_outputs(cgh, gpu):
+    """Output written while the stripes arrive: to a file, gzip (parallel
+    members) by -z or by a .gz name, multi-stripe, every layout; the
+    decompressed text must match the oracle cell for cell."""
+    import gzip
+    n, L, seed, hashed = oracle.CHECK_CASES[-1]
+    A, cells = oracle.check_make(n, L, seed, hashed)
+    H = oracle.check_ref(A)
+    iu = np.triu(np.ones_like(H, dtype=bool))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "in.tsv")
+        oracle.write_check_tsv(path, cells, L)
+        runs = [
+            (["-z", "-o", os.path.join(d, "a")], True, H),
+            (["-o", os.path.join(d, "b.gz"), "--stripe", "64", "-m", "upper-tri"], True, np.where(iu, H, 0)),
+            (["-o", os.path.join(d, "c.tsv"), "--stripe", "8", "-m", "lower-tri", "-X", "9"], False,
+             np.minimum(np.where(iu.T, H, 0), 9)),
+        ]
+        for args, zipped, want in runs:
+            print(f"cli outputs args={args[2:]}", flush=True)
+            out = subprocess.run([cgh.CLI_PATH, "-s", "-i", path, *args], capture_output=True, text=True, timeout=120)
+            assert out.returncode == 0, out.stderr
+            target = args[args.index("-o") + 1]
+            raw = open(target, "rb").read()
+            assert (raw[:2] == b"\x1f\x8b") == zipped
+            text = (gzip.decompress(raw) if zipped else raw).decode()
+            assert np.array_equal(oracle.parse_cli_output(text), want), args
+
+
 @pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("tau", [1, 2, 8, 64, 4096])
 @pytest.mark.parametrize("S", [64, 512, 2048])
