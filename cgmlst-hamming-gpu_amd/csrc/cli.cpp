@@ -18,6 +18,7 @@
 
 #include <chrono>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "cgmlst_hamming.h"
@@ -346,15 +347,46 @@ bool read_all(const std::string& path, std::string* out, int* eno) {
     return ok;
 }
 
-struct Upper {
-    std::vector<std::vector<uint16_t>> rows;  // row i: H[i][i+1..n-1]
+// The symmetric matrix as it arrives: row i of stripe s is copied in (its
+// strict upper part) and mirrored into column i of the later rows, so that
+// every row is contiguous by the time it is formatted.
+struct Matrix {
+    size_t n = 0;
+    std::unique_ptr<uint16_t[]> m;  // n x n; pages are touched first by the (parallel) stripe copies
+    void init(size_t rows) {
+        n = rows;
+        m.reset(new uint16_t[std::max<size_t>(n * n, 1)]);
+    }
+    void zero() {
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < (long)n; i++) memset(row((size_t)i), 0, n * 2);
+    }
+    uint16_t* row(size_t i) { return m.get() + i * n; }
+    const uint16_t* row(size_t i) const { return m.get() + i * n; }
 };
 
-inline uint16_t at(const Upper& u, size_t i, size_t j) {
-    if (i == j) return 0;
-    if (j > i) return u.rows[i][j - i - 1];
-    return u.rows[j][i - j - 1];
-}
+// Distance -> "<sep><decimal>" in 16-byte slots (sep of at most 10 bytes),
+// so a cell is one table copy.
+struct CellText {
+    std::vector<char> slot;   // 16 bytes per value
+    std::vector<uint8_t> len;
+    std::string osep;
+    bool table = false;
+    void build(const std::string& sep, uint32_t maxv) {
+        osep = sep;
+        table = sep.size() <= 10;
+        if (!table) return;
+        slot.assign((size_t)(maxv + 1) * 16, 0);
+        len.assign((size_t)maxv + 1, 0);
+        for (uint32_t v = 0; v <= maxv; v++) {
+            char d[12];
+            const int k = snprintf(d, sizeof d, "%u", v);
+            memcpy(&slot[(size_t)v * 16], sep.data(), sep.size());
+            memcpy(&slot[(size_t)v * 16 + sep.size()], d, (size_t)k);
+            len[v] = (uint8_t)(sep.size() + (size_t)k);
+        }
+    }
+};
 
 char* fmt_u(char* p, uint32_t v) {
     char tmp[12];
@@ -374,9 +406,10 @@ char* fmt_u(char* p, uint32_t v) {
 // (level 3), compressed in parallel too: concatenated members are one valid
 // gzip stream.  The file is opened at the first write.
 struct Writer {
-    static constexpr size_t GROUP = 64;
+    static constexpr size_t GROUP = 16;
     const std::vector<std::string>* ids = NULL;
-    const Upper* up = NULL;
+    const Matrix* mx = NULL;
+    CellText cells;
     std::string osep, index_name, path;
     Layout layout = Layout::Full;
     int32_t cap = 0;
@@ -442,17 +475,30 @@ struct Writer {
 
     void format_row(size_t i, std::string& b) const {
         const size_t n = ids->size();
-        b.resize((*ids)[i].size() + n * (osep.size() + 11) + 1);
+        b.resize((*ids)[i].size() + n * (osep.size() + 11) + 16);
         char* p = &b[0];
         memcpy(p, (*ids)[i].data(), (*ids)[i].size());
         p += (*ids)[i].size();
-        for (size_t j = 0; j < n; j++) {
-            memcpy(p, osep.data(), osep.size());
-            p += osep.size();
-            const bool keep = layout == Layout::Full || (layout == Layout::LowerTri ? j <= i : j >= i);
-            int32_t v = keep ? (int32_t)at(*up, i, j) : 0;
-            if (cap > 0 && v > cap) v = cap;
-            p = fmt_u(p, (uint32_t)v);
+        const uint16_t* r = mx->row(i);
+        // kept columns [j0, j1): the rest print 0 (main.rs:245-253)
+        const size_t j0 = layout == Layout::UpperTri ? i : 0, j1 = layout == Layout::LowerTri ? i + 1 : n;
+        const uint32_t c = cap > 0 ? (uint32_t)cap : 0xffffffffu;
+        if (cells.table) {
+            const char* zero = &cells.slot[0];
+            const size_t zl = cells.len[0];
+            for (size_t j = 0; j < j0; j++, p += zl) memcpy(p, zero, 16);
+            for (size_t j = j0; j < j1; j++) {
+                const uint32_t v = std::min<uint32_t>(r[j], c);
+                memcpy(p, &cells.slot[(size_t)v * 16], 16);
+                p += cells.len[v];
+            }
+            for (size_t j = j1; j < n; j++, p += zl) memcpy(p, zero, 16);
+        } else {
+            for (size_t j = 0; j < n; j++) {
+                memcpy(p, osep.data(), osep.size());
+                p += osep.size();
+                p = fmt_u(p, j >= j0 && j < j1 ? std::min<uint32_t>(r[j], c) : 0u);
+            }
         }
         *p++ = '\n';
         b.resize((size_t)(p - &b[0]));
@@ -493,18 +539,36 @@ struct Writer {
 };
 
 struct StripeCtx {
-    Upper* up;
+    Matrix* mx;
     Writer* w;
 };
 
-// Stripe callback (tch_cb): keep the strict upper rows, then write every row
-// of the stripe while the device computes the next one.
+// Stripe callback (tch_cb): rows r0.. of the stripe (columns j = r0 + c,
+// c > r kept) into the matrix and mirrored into the later rows in 64 x 64
+// blocks, then every row of the stripe is written while the device computes
+// the next stripe.
 void on_stripe(void* ctx, int r0, int rows, size_t w, const uint16_t* stripe) {
     StripeCtx* c = (StripeCtx*)ctx;
+    Matrix& M = *c->mx;
+    const size_t n = M.n, R0 = (size_t)r0;
 #pragma omp parallel for schedule(static)
-    for (int r = 0; r < rows; r++)
-        c->up->rows[(size_t)r0 + r].assign(stripe + (size_t)r * w + r + 1, stripe + (size_t)(r + 1) * w);
-    c->w->rows_until((size_t)r0 + rows);
+    for (int r = 0; r < rows; r++) {
+        M.row(R0 + r)[R0 + r] = 0;
+        memcpy(M.row(R0 + r) + R0 + r + 1, stripe + (size_t)r * w + r + 1, (w - r - 1) * 2);
+    }
+    const size_t jb = (n - R0 + 63) / 64;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long b = 0; b < (long)jb; b++) {
+        const size_t j0 = R0 + (size_t)b * 64, j1 = std::min(n, j0 + 64);
+        for (size_t i0 = R0; i0 < R0 + rows; i0 += 64) {
+            const size_t i1 = std::min(R0 + rows, i0 + 64);
+            for (size_t j = j0; j < j1; j++) {
+                uint16_t* dst = M.row(j);
+                for (size_t i = i0; i < i1 && i < j; i++) dst[i] = M.row(i)[j];
+            }
+        }
+    }
+    c->w->rows_until(R0 + rows);
 }
 
 void log_line(void* ctx, const char* line) {
@@ -644,9 +708,10 @@ int main(int argc, char** argv) {
     // ---- encode (main.rs:353-445), on the device ----
     t = now();
     char err[512] = {0};
-    Upper up;
-    up.rows.resize(n);
-    wr.up = &up;
+    Matrix mx;
+    mx.init(n);  // 7.9 GB at 63k: every cell is written before its row is formatted
+    wr.mx = &mx;
+    wr.cells.build(args.output_sep, (uint32_t)std::max<size_t>(l, 1));
     cgh_dataset* ds = NULL;
     const size_t stripe = std::max<size_t>(args.stripe, 8) / 8 * 8;
     if (n >= 2 && l > 0) {
@@ -668,19 +733,18 @@ int main(int argc, char** argv) {
         if (cgh_dense_ranks(calls.data(), (int)n, (int)l, X.data(), err, sizeof err)) die(err);
     }
     std::vector<uint64_t>().swap(calls);
+    if (!ds) mx.zero();  // nothing computed: an all-zero matrix (main.rs:466-470)
     if (ds) {
         // ---- GPU (main.rs:447-465) ----
         t = now();
         double gpu_secs = 0;
-        StripeCtx sc = {&up, &wr};
+        StripeCtx sc = {&mx, &wr};
         const int rc = cgh_dataset_stream(ds, (int)stripe, on_stripe, &sc, &gpu_secs, err, sizeof err);
         cgh_dataset_free(ds);
         if (rc) die(std::string("GPU: ") + err);
         snprintf(msg, sizeof msg, "Computed %zu x %zu matrix on MFMA in %.2fs (GPU pipeline %.2fs, rare pairs on GPU)",
                  n, n, now() - t, gpu_secs);
         log(msg);
-    } else {
-        for (size_t i = 0; i < n; i++) up.rows[i].assign(n - 1 - i, 0);
     }
 
     // ---- the rows not written yet (all of them when nothing ran on the device) ----
