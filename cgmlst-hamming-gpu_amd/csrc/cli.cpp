@@ -725,6 +725,7 @@ int main(int argc, char** argv) {
         size_t D = 0;
         uint64_t n_rare = 0;
         cgh_dataset_info(ds, NULL, NULL, NULL, &D, &n_rare);
+        if (args.max_dist > 0) cgh_dataset_set_cap(ds, args.max_dist);  // -X on the device
         snprintf(msg, sizeof msg, "Encoded in %.2fs: tau=%d, one-hot cols=%zu, rare equal pairs=%llu", now() - t,
                  tau_used, D, (unsigned long long)n_rare);
         log(msg);

@@ -79,7 +79,19 @@ struct TileJob {
     const uint32_t* PB;
     int nB, rect;
     int ldA, ldB;       // rows per plane of (O, P) and of (OB, PB)
+    // Cap (main.rs:251-253: a value above cap is written as cap; 0 = none).
+    // Rows with rare partners are capped by the rare pass, after the
+    // subtraction; the epilogue caps the others (rstart[i + 1] == rstart[i]).
+    const uint32_t* rstart;
+    int cap;
 };
+
+// Cell value of row i (global) in the epilogue, capped unless the rare pass
+// still has to subtract from that row.
+__device__ __forceinline__ uint16_t epilogue_cap(const TileJob& a, int i, uint32_t v) {
+    if (a.cap > 0 && (!a.rstart || a.rstart[i + 1] == a.rstart[i])) v = min(v, (uint32_t)a.cap);
+    return (uint16_t)v;
+}
 
 struct TileArgs {
     TileJob job;
@@ -363,19 +375,30 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
 
     // Epilogue: lane holds column l32 and rows 8*(e/4) + 4*g + (e%4) of
-    // O O^T - P P^T; the distance is its negation.
+    // O O^T - P P^T; the distance is its negation.  The capped form is a
+    // separate loop so that the common one stays a plain store sequence.
+    auto store_all = [&](auto capped) {
 #pragma unroll
-    for (int i = 0; i < TM; i++)
+        for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++) {
-            const int c = ct0 + wn0 + j * 32 + l32;
-            if (c >= w) continue;
+            for (int j = 0; j < TN; j++) {
+                const int c = ct0 + wn0 + j * 32 + l32;
+                if (c >= w) continue;
 #pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const int r = rt0 + wm0 + i * 32 + (e >> 2) * 8 + g * 4 + (e & 3);
-                if (r < rows) out[(size_t)r * w + c] = (uint16_t)(a.rect || c > r ? -(int)acc[i][j][e] : 0);
+                for (int e = 0; e < 16; e++) {
+                    const int r = rt0 + wm0 + i * 32 + (e >> 2) * 8 + g * 4 + (e & 3);
+                    if (r >= rows) continue;
+                    const uint32_t v = (uint32_t)(-(int)acc[i][j][e]);
+                    uint16_t o;
+                    if (a.rect) o = (uint16_t)v;
+                    else if constexpr (decltype(capped)::value) o = c > r ? epilogue_cap(a, sr0 + r, v) : 0;
+                    else o = c > r ? (uint16_t)v : 0;
+                    out[(size_t)r * w + c] = o;
+                }
             }
-        }
+    };
+    if (a.cap > 0) store_all(std::true_type{});
+    else store_all(std::false_type{});
 }
 
 // Tile shapes used: 256 x 256 by 8 wavefronts of 128 x 64 (two per SIMD, so
@@ -428,6 +451,8 @@ __device__ __forceinline__ void scalarize(TileJob& j) {
     j.rect = __builtin_amdgcn_readfirstlane(j.rect);
     j.ldA = __builtin_amdgcn_readfirstlane(j.ldA);
     j.ldB = __builtin_amdgcn_readfirstlane(j.ldB);
+    j.rstart = uniform_ptr(j.rstart);
+    j.cap = __builtin_amdgcn_readfirstlane(j.cap);
 }
 
 // Persistent form for many datasets: each workgroup pulls tiles (job,
@@ -554,6 +579,7 @@ __device__ __forceinline__ void valu_sub(const TileJob a, int sr0, int rt0, int 
     }
 #undef VALU_PIECE
 #undef VALU_STORE
+    const bool capped = a.cap > 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         const int r = rt0 + ty + 16 * i;
@@ -561,7 +587,10 @@ __device__ __forceinline__ void valu_sub(const TileJob a, int sr0, int rt0, int 
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int cc = ct0 + tx + 16 * j;
-            if (cc < w) out[(size_t)r * w + cc] = (uint16_t)(a.rect || cc > r ? acc[i][j] : 0u);
+            if (cc >= w) continue;
+            const uint32_t v = acc[i][j];
+            out[(size_t)r * w + cc] = a.rect ? (uint16_t)v
+                                    : cc > r ? (capped ? epilogue_cap(a, sr0 + r, v) : (uint16_t)v) : (uint16_t)0;
         }
     }
 }
@@ -675,6 +704,7 @@ struct RareJob {
     int d;                     // dataset of this job in a batched launch
     int rect, nB;              // rectangular form: rows against nB columns
     int m16;
+    int cap;                   // > 0: staged cells are written back as min(value, cap)
 };
 
 struct RareShared {
@@ -763,6 +793,11 @@ __device__ __forceinline__ void rare_rows_impl(const RareJob& jb, const M* __res
         __syncthreads();  // ebeg / eend are rewritten by the next batch
     }
 
+    if (jb.cap > 0) {  // main.rs:251-253 on the final values
+        uint16_t* c16 = reinterpret_cast<uint16_t*>(cells4);
+        for (int k = tid; k < nq * 8; k += RNT) c16[k] = min(c16[k], (uint16_t)jb.cap);
+        __syncthreads();
+    }
     uint4* __restrict__ dst = reinterpret_cast<uint4*>(out + a0);
     for (int k = tid; k < nq; k += RNT) {
         const size_t lo = a0 + 8 * (size_t)k;  // u16 index of the word's first cell
@@ -926,11 +961,13 @@ hipError_t upload_members(const uint32_t* mem, size_t count, bool m16, void** ou
 }
 
 hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int ld, int n,
-                        int r0, int S, int ns, uint16_t* out, hipStream_t st) {
+                        int r0, int S, int ns, uint16_t* out, hipStream_t st, const uint32_t* rstart = NULL,
+                        int cap = 0) {
     if (ns > 1 && S % tile) {
         for (int s = 0; s < ns; s++) {
             const hipError_t e = launch_tiles(kernel, tile, O, kw1, P, kw2, ld, n, r0 + s * S, S, 1,
-                                              out + stripe_off((size_t)(n - r0), (size_t)S, (size_t)s), st);
+                                              out + stripe_off((size_t)(n - r0), (size_t)S, (size_t)s), st, rstart,
+                                              cap);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -940,6 +977,7 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
     a.job.out = out;
     a.job.OB = O; a.job.PB = P; a.job.nB = n; a.job.rect = 0;
     a.job.ldA = ld; a.job.ldB = ld;
+    a.job.rstart = rstart; a.job.cap = cap;
     a.ns = ns;
     a.nbc = (n - r0 + tile - 1) / tile;
     a.q = S / tile;
@@ -1091,6 +1129,7 @@ struct cgh_dataset {
     int device;
     int n, L, tau;
     size_t D;                 // one-hot columns
+    int cap;                  // > 0: outputs capped (cgh_dataset_set_cap)
     size_t n_pad, kw1, kw2;   // kw1, kw2: words per row on the device (multiples of KC)
     uint32_t *dO, *dP;        // planes of n_pad rows
     size_t n_groups, n_members;
@@ -1264,6 +1303,12 @@ hipError_t launch_build_planes(const uint16_t* dX, int n, int L, const uint32_t*
 
 }  // namespace cghi
 
+extern "C" int cgh_dataset_set_cap(cgh_dataset* ds, int cap) {
+    if (cap < 0) return 1;
+    ds->cap = cap;
+    return 0;
+}
+
 extern "C" void cgh_dataset_info(const cgh_dataset* ds, int* n, int* L, int* tau, size_t* D, uint64_t* n_rare) {
     if (n) *n = ds->n;
     if (L) *L = ds->L;
@@ -1305,10 +1350,10 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
     HK(hipSetDevice(ds->device));
     if (phases & CGH_PHASE_PRODUCT)
         HK(launch_tiles(resolve_kernel(kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
-                         S, (n + S - 1) / S, d_out, st));
+                         S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         const int chunks = (n - 1 + RCW - 1) / RCW;
-        const RareJob jb = {d_out, n, 0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16};
+        const RareJob jb = {d_out, n, 0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16, ds->cap};
         hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)n * chunks)), dim3(RNT), 0, st, jb);
         HK(hipGetLastError());
     }
@@ -1389,7 +1434,7 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         b->offs.push_back(b->total);
         b->total += cgh_upper_elems(ds->n, S);
         jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL, ds->dO, ds->dP, ds->n, 0,
-                   (int)ds->n_pad, (int)ds->n_pad};
+                   (int)ds->n_pad, (int)ds->n_pad, ds->drstart, ds->cap};
         const int k = tl == 256 ? 0 : 1;
         const long long cost = (long long)(ds->kw1 + ds->kw2);
         for (int s = 0, r0 = 0; r0 < ds->n; s++, r0 += S) {
@@ -1399,7 +1444,7 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         }
         if (ds->n_rare && ds->n > 1) {
             RareJob r = {NULL, ds->n, 0, S, (ds->n - 1 + RCW - 1) / RCW, ds->drstart, ds->drentry, ds->dgmem,
-                         b->rare_blocks, d, 0, 0, ds->m16};
+                         b->rare_blocks, d, 0, 0, ds->m16, ds->cap};
             rjobs.push_back(r);
             b->rare_blocks += (long long)ds->n * r.chunks;
         }
@@ -1751,7 +1796,7 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     const int tile = q > 128 ? 256 : 128;
     TileArgs a;
     a.job = {sl.dO, sl.dP, (int)db->ds->kw1, (int)db->ds->kw2, q, 0, q, sl.ddist, db->ds->dO, db->ds->dP, db->n, 1,
-             q_pad, (int)db->ds->n_pad};
+             q_pad, (int)db->ds->n_pad, NULL, 0};
     a.ns = 1;
     a.q = 0;
     a.nbc = (db->n + tile - 1) / tile;
@@ -1769,7 +1814,7 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);
     if (ne) {
         const int chunks = (db->n + RCW - 1) / RCW;
-        const RareJob jb = {sl.ddist, q, 0, q, chunks, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n, db->m16};
+        const RareJob jb = {sl.ddist, q, 0, q, chunks, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n, db->m16, 0};
         hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)q * chunks)), dim3(RNT), 0, sl.st, jb);
     }
     const int tchunks = (db->n + TKC - 1) / TKC;
@@ -2051,12 +2096,11 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
             const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
             const size_t w = (size_t)(n - r0);
             HK(launch_tiles(resolve_kernel(CGH_KERNEL_AUTO), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2,
-                            (int)ds->n_pad, n, r0,
-                            S, 1, dU[b], st[b]));
+                            (int)ds->n_pad, n, r0, S, 1, dU[b], st[b], ds->drstart, ds->cap));
             if (ds->n_rare && n - r0 > 1) {
                 const int chunks = (n - r0 - 1 + RCW - 1) / RCW;
                 const RareJob jb = {dU[b], n, r0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0,
-                                    ds->m16};
+                                    ds->m16, ds->cap};
                 hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)rows * chunks)), dim3(RNT), 0, st[b], jb);
                 HK(hipGetLastError());
             }

@@ -164,6 +164,13 @@ int cgh_dataset_encode(int device, const uint64_t* calls, int n, int L, int tau,
                        void (*log)(void* ctx, const char* line), void* log_ctx, cgh_dataset** out, int* tau_used,
                        uint16_t* X, char* err, int errlen);
 
+/* Cap every later output of the dataset (upper, phase = both phases, stream,
+ * batch): a value above cap is written as cap, cap = 0 (the default) none;
+ * main.rs:251-253 applied on the device (in the product epilogue for rows
+ * without rare pairs, in the rare pass for the others).  Returns 1 if
+ * cap < 0. */
+int cgh_dataset_set_cap(cgh_dataset* ds, int cap);
+
 /* Sizes of a dataset: profiles, loci, tau, one-hot columns, rare equal
  * pairs (any pointer may be NULL). */
 void cgh_dataset_info(const cgh_dataset* ds, int* n, int* L, int* tau, size_t* D, uint64_t* n_rare);

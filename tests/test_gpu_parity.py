@@ -20,10 +20,12 @@ pytestmark = pytest.mark.gpu
 KERNELS = [1, 2, 3]  # CGH_KERNEL_MFMA_I8, CGH_KERNEL_MFMA_FP4, CGH_KERNEL_VALU_POPC
 
 
-def gpu_upper(cgh, torch, X, tau, S, k_align=256, kernel=0):
+def gpu_upper(cgh, torch, X, tau, S, k_align=256, kernel=0, cap=0):
     """Encode X at tau, upload, run cgh_dataset_upper, return the square H."""
     enc = cgh.Encoded(X, tau, k_align)
     ds = cgh.Dataset(enc)
+    if cap:
+        ds.set_cap(cap)
     n = X.shape[0]
     out = torch.full((cgh.upper_elems(n, S),), 0x5a5a, dtype=torch.int16, device="cuda")
     st = torch.cuda.current_stream()
@@ -81,6 +83,19 @@ def test_cli_gzip_and_file_outputs(cgh, gpu):
             assert (raw[:2] == b"\x1f\x8b") == zipped
             text = (gzip.decompress(raw) if zipped else raw).decode()
             assert np.array_equal(oracle.parse_cli_output(text), want), args
+
+
+@pytest.mark.parametrize("kernel", [2, 3])
+@pytest.mark.parametrize("tau", [1, 64, 4096])
+@pytest.mark.parametrize("cap", [1, 9, 400])
+def test_dataset_cap_on_device(cgh, gpu, workloads, tau, cap, kernel):
+    """-X on the device: min(H, cap) for every cell, whether the row is
+    finished by the product epilogue (no rare partners) or by the rare pass."""
+    X = cgh.dense_ranks(workloads.small_zipf(1200, 500, seed=3))
+    H_ref = oracle.brute_full(X)
+    print(f"dataset cap={cap} tau={tau} kernel={kernel}", flush=True)
+    H, _, _ = gpu_upper(cgh, gpu, X, tau, 256, kernel=kernel, cap=cap)
+    assert np.array_equal(H, np.minimum(H_ref, cap))
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
