@@ -27,7 +27,7 @@ EXPORTS = (
     "tch_device_count", "tch_free_vram", "tch_run",
     "cgh_dense_ranks", "cgh_encode", "cgh_encode_meta", "cgh_column_table", "cgh_rare_csr", "cgh_encoding_free",
     "cgh_allele_counts", "cgh_choose_tau",
-    "cgh_dataset_upload", "cgh_dataset_build", "cgh_dataset_encode", "cgh_dataset_set_cap", "cgh_dataset_info", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
+    "cgh_dataset_upload", "cgh_dataset_build", "cgh_dataset_encode", "cgh_dataset_set_cap", "cgh_dataset_autotune", "cgh_dataset_info", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
     "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream",
     "cgh_batch_create", "cgh_batch_out_elems", "cgh_batch_out_offset", "cgh_batch_tile", "cgh_batch_run",
     "cgh_batch_free",
@@ -83,6 +83,7 @@ def load():
         "cgh_dataset_build": (I, [I, P, I, I, I, C.POINTER(P), C.c_char_p, I]),
         "cgh_dataset_encode": (I, [I, P, I, I, I, S, P, P, C.POINTER(P), C.POINTER(I), P, C.c_char_p, I]),
         "cgh_dataset_set_cap": (I, [P, I]),
+        "cgh_dataset_autotune": (I, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_double), C.c_char_p, I]),
         "cgh_dataset_info": (None, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(S), C.POINTER(U64)]),
         "cgh_rare_csr": (I, [C.POINTER(Encoding), P, P, C.c_char_p, I]),
         "cgh_encoding_free": (None, [C.POINTER(Encoding)]),
@@ -293,6 +294,13 @@ class Dataset:
                                          C.byref(ds.handle), C.byref(t), _ptr(X) if X is not None else None, err,
                                          512), err)
         return ds, t.value, X
+
+    def autotune(self, S: int):
+        """cgh_dataset_autotune -> (kernel, tile, ms of the fastest)"""
+        k, t, ms = C.c_int(), C.c_int(), C.c_double()
+        err = _err()
+        _check(load().cgh_dataset_autotune(self.handle, S, C.byref(k), C.byref(t), C.byref(ms), err, 512), err)
+        return k.value, t.value, ms.value
 
     def set_cap(self, cap: int):
         """Outputs from now on: values above cap written as cap (0 = none)."""

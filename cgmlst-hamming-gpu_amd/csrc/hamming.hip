@@ -1130,6 +1130,7 @@ struct cgh_dataset {
     int n, L, tau;
     size_t D;                 // one-hot columns
     int cap;                  // > 0: outputs capped (cgh_dataset_set_cap)
+    int tuned_kernel, tuned_tile;  // from cgh_dataset_autotune (0 = not tuned)
     size_t n_pad, kw1, kw2;   // kw1, kw2: words per row on the device (multiples of KC)
     uint32_t *dO, *dP;        // planes of n_pad rows
     size_t n_groups, n_members;
@@ -1326,15 +1327,82 @@ extern "C" size_t cgh_upper_elems(int n, int S) {
     return up(stripe_off((size_t)n, (size_t)S, (size_t)(ns - 1)) + last * (size_t)(n - (ns - 1) * S), 8);
 }
 
-extern "C" int cgh_dataset_tile(const cgh_dataset* ds, int kernel) {
-    (void)kernel;
-    return pick_tile(ds->n);
+namespace {
+
+// The dataset's product kernel and tile: the autotuned pair when there is
+// one (for CGH_KERNEL_AUTO), else fp4 MFMA and 256-tiles from n >= 8192.
+int ds_kernel(const cgh_dataset* ds, int kernel) {
+    return kernel == CGH_KERNEL_AUTO && ds->tuned_kernel ? ds->tuned_kernel : resolve_kernel(kernel);
+}
+int ds_tile(const cgh_dataset* ds, int kernel) {
+    return ds->tuned_tile && ds_kernel(ds, kernel) == ds->tuned_kernel ? ds->tuned_tile : pick_tile(ds->n);
+}
+
+}  // namespace
+
+extern "C" int cgh_dataset_tile(const cgh_dataset* ds, int kernel) { return ds_tile(ds, kernel); }
+
+extern "C" int cgh_dataset_autotune(cgh_dataset* ds, int S, int* kernel_out, int* tile_out, double* ms_out,
+                                    char* err, int errlen) {
+    // Candidates: fp4 and int8 MFMA x 128- and 256-tiles (the VALU form is
+    // never faster), each timed on the first stripe (S rows, the widest)
+    // after one warm-up launch; the fastest is kept.
+    const int n = ds->n;
+    const int ns = 1;
+    const int kernels[2] = {CGH_KERNEL_MFMA_FP4, CGH_KERNEL_MFMA_I8}, tiles[2] = {256, 128};
+    uint16_t* buf = NULL;
+    hipStream_t st = NULL;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    double best = 1e300;
+    int bk = 0, bt = 0;
+    if (S < 8 || S % 8) {
+        set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
+        return 1;
+    }
+    HK(hipSetDevice(ds->device));
+    HK(hipMalloc(&buf, up(stripe_off((size_t)n, (size_t)S, (size_t)ns), 8) * 2 + 16));
+    HK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HK(hipEventCreate(&e0));
+    HK(hipEventCreate(&e1));
+    for (int k : kernels)
+        for (int t : tiles) {
+            float ms = 0.f;
+            HK(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf, st));
+            HK(hipEventRecord(e0, st));
+            for (int rep = 0; rep < 3; rep++)
+                HK(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf,
+                                st));
+            HK(hipEventRecord(e1, st));
+            HK(hipEventSynchronize(e1));
+            HK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / 3 < best) {
+                best = ms / 3;
+                bk = k;
+                bt = t;
+            }
+        }
+    ds->tuned_kernel = bk;
+    ds->tuned_tile = bt;
+    if (kernel_out) *kernel_out = bk;
+    if (tile_out) *tile_out = bt;
+    if (ms_out) *ms_out = best;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(buf);
+    return 0;
+fail:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (buf) (void)hipFree(buf);
+    return 1;
 }
 
 extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, int phases,
                                  char* err, int errlen) {
     hipStream_t st = (hipStream_t)stream;
-    const int n = ds->n, tile = pick_tile(n);
+    const int n = ds->n, tile = ds_tile(ds, kernel);
     if (S < 8 || S % 8) {
         set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
         return 1;
@@ -1349,7 +1417,7 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
     }
     HK(hipSetDevice(ds->device));
     if (phases & CGH_PHASE_PRODUCT)
-        HK(launch_tiles(resolve_kernel(kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
+        HK(launch_tiles(ds_kernel(ds, kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
                          S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         const int chunks = (n - 1 + RCW - 1) / RCW;
@@ -1429,6 +1497,8 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         }
         // per-dataset tile: 256 when its triangle holds enough 256-tiles to
         // fill the chip, else 128
+        // (a dataset's own tuned tile is not used: inside one persistent
+        // launch 256-tiles pay off from smaller n than in a launch of its own)
         const int tl = tile ? tile : (ds->n >= 4096 ? 256 : 128);
         b->tile.push_back(tl);
         b->offs.push_back(b->total);
@@ -2075,7 +2145,7 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
     hipEvent_t done[2] = {NULL, NULL}, t0 = NULL, t1 = NULL;
     int rc = 1, prev = -1;
     float ms = 0.f;
-    const int n = ds->n, tile = pick_tile(n);
+    const int n = ds->n, tile = ds_tile(ds, CGH_KERNEL_AUTO);
     const size_t stripe_elems = up((size_t)S * ds->n_pad, 8);
     if (S < 8 || S % 8) {
         set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
@@ -2095,7 +2165,7 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
         if (r0 < n) {
             const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
             const size_t w = (size_t)(n - r0);
-            HK(launch_tiles(resolve_kernel(CGH_KERNEL_AUTO), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2,
+            HK(launch_tiles(ds_kernel(ds, CGH_KERNEL_AUTO), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2,
                             (int)ds->n_pad, n, r0, S, 1, dU[b], st[b], ds->drstart, ds->cap));
             if (ds->n_rare && n - r0 > 1) {
                 const int chunks = (n - r0 - 1 + RCW - 1) / RCW;

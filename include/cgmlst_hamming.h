@@ -171,6 +171,15 @@ int cgh_dataset_encode(int device, const uint64_t* calls, int n, int L, int tau,
  * cap < 0. */
 int cgh_dataset_set_cap(cgh_dataset* ds, int cap);
 
+/* Autotuner: time the product kernel candidates (fp4 and int8 MFMA, 128- and
+ * 256-tiles) on the dataset's first stripe of S rows and keep the fastest
+ * for CGH_KERNEL_AUTO in every later call on this dataset (upper, phase,
+ * stream; a batch keeps its own tile rule).  Outputs
+ * (may be NULL): the kernel (CGH_KERNEL_*), the tile edge and its time in
+ * ms.  No counterpart in the reference (its CUTLASS tile is fixed at build
+ * time, tensorcore/README.md:31-32). */
+int cgh_dataset_autotune(cgh_dataset* ds, int S, int* kernel, int* tile, double* ms, char* err, int errlen);
+
 /* Sizes of a dataset: profiles, loci, tau, one-hot columns, rare equal
  * pairs (any pointer may be NULL). */
 void cgh_dataset_info(const cgh_dataset* ds, int* n, int* L, int* tau, size_t* D, uint64_t* n_rare);

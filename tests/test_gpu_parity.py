@@ -136,6 +136,25 @@ def test_dataset_tile256(cgh, gpu, kernel):
     assert np.array_equal(H, oracle.brute_full(X))
 
 
+@pytest.mark.parametrize("shape", [(1500, 700, 64), (9000, 200, 4096)])
+def test_autotune_then_exact(cgh, gpu, workloads, shape):
+    """The autotuner picks one of the timed candidates and the dataset then
+    runs with it (CGH_KERNEL_AUTO), every cell exact."""
+    n, L, S = shape
+    X = cgh.dense_ranks(workloads.small_zipf(n, L, seed=11))
+    ds = cgh.Dataset(cgh.Encoded(X, 32, 256))
+    print(f"autotune n={n} S={S}", flush=True)
+    k, t, ms = ds.autotune(S)
+    assert k in (1, 2) and t in (128, 256) and ms > 0
+    assert ds.tile(0) == t
+    out = gpu.zeros(cgh.upper_elems(n, S), dtype=gpu.int16, device="cuda")
+    ds.upper(S, out.data_ptr(), gpu.cuda.current_stream().cuda_stream, 0)
+    gpu.cuda.synchronize()
+    H = cgh.upper_to_square(out.cpu().numpy().view(np.uint16), n, S)
+    assert np.array_equal(H, oracle.brute_full(X))
+    ds.close()
+
+
 @pytest.mark.parametrize("force_host", [False, True])
 def test_tch_run(cgh, gpu, monkeypatch, force_host):
     """The reference-compatible tch_run entry point, rare pairs on the GPU
