@@ -693,7 +693,15 @@ constexpr int RCW = 65536;  // u16 cells per chunk (128 KiB of LDS: one workgrou
 constexpr int RNT = 1024;   // threads per workgroup = ranges per batch
 constexpr int RQ = (RCW / 8 + 1 + RNT - 1) / RNT;  // 16-byte words staged per thread
 constexpr int RUN = 8;      // partners per lane per pass
-constexpr int RSG = 32;     // lanes per range
+// Lanes per range: about 0.4 x the mean range length (partners per range),
+// a power of two in [4, 64]: short ranges leave fewer lanes idle, long ones
+// keep fewer ranges in flight.
+int lanes_per_range(uint64_t pairs, size_t ranges) {
+    const double mean = ranges ? (double)pairs / (double)ranges : 1.0;
+    int g = 4;
+    while (g < 64 && g * 2 <= mean * 0.4) g *= 2;
+    return g;
+}
 struct RareJob {
     uint16_t* out;             // stripe layout of rows r0..
     int n, r0, S, chunks;      // chunks = column chunks per row
@@ -705,6 +713,11 @@ struct RareJob {
     int rect, nB;              // rectangular form: rows against nB columns
     int m16;
     int cap;                   // > 0: staged cells are written back as min(value, cap)
+    // Row groups: G > 1 consecutive rows of one stripe per workgroup (rows
+    // short enough that G of them fit one chunk), gps groups per stripe;
+    // G = 1: one row per workgroup, in `chunks` column chunks.
+    int G, gps;
+    int rsg;                   // lanes per range (a power of two, 4 .. 64)
 };
 
 struct RareShared {
@@ -713,9 +726,13 @@ struct RareShared {
     uint32_t eend[RNT];
 };
 
-// Row i (>= r0), column chunk ch of job jb, members of type M.
+// Rows i0 .. i1-1 (>= r0, one stripe) of job jb, members of type M: with
+// one row, column chunk ch of it; with several, every cell j > i of each row
+// (they fit one chunk).  The cells are staged as one contiguous range of the
+// layout (for several rows it includes their cells j <= i, which no partner
+// touches and which are written back unchanged).
 template <typename M>
-__device__ __forceinline__ void rare_rows_impl(const RareJob& jb, const M* __restrict__ gmem, int i, int ch,
+__device__ __forceinline__ void rare_rows_impl(const RareJob& jb, const M* __restrict__ gmem, int i0, int i1, int ch,
                                                RareShared& sh) {
     uint16_t* __restrict__ out = jb.out;
     const int n = jb.n, r0 = jb.r0, S = jb.S, chunks = jb.chunks;
@@ -726,18 +743,22 @@ __device__ __forceinline__ void rare_rows_impl(const RareJob& jb, const M* __res
     uint32_t* eend = sh.eend;
     uint32_t* cells = reinterpret_cast<uint32_t*>(cells4);
     const int tid = threadIdx.x;
-    const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
+    if (rstart[i0] == rstart[i1]) return;  // no rare partners in these rows
     // symmetric: cells j in (i, n) of the stripe layout of rows r0..
     // (stripe s = rows r0+s*S ..); rectangular: cells j in [0, nB) of row
     // i - r0 of an (n - r0) x nB row-major matrix.  u16 index of (i, j) is
-    // base + j.
-    const int jlo = jb.rect ? 0 : i + 1, jhi = jb.rect ? jb.nB : n;
-    const int j0 = jlo + ch * RCW, j1 = min(jhi, j0 + RCW);
-    if (e0 == e1 || j0 >= jhi) return;
-    const size_t s = jb.rect ? 0 : (size_t)((i - r0) / S), sr0 = (size_t)r0 + s * S;
-    const size_t base = jb.rect ? (size_t)(i - r0) * (size_t)jb.nB
-                                : stripe_off((size_t)(n - r0), (size_t)S, s) + ((size_t)i - sr0) * ((size_t)n - sr0) - sr0;
-    const size_t g0 = base + j0, g1 = base + j1;   // cells [g0, g1)
+    // row_base(i) + j.
+    const size_t s = jb.rect ? 0 : (size_t)((i0 - r0) / S), sr0 = (size_t)r0 + s * S;
+    const size_t sbase = jb.rect ? 0 : stripe_off((size_t)(n - r0), (size_t)S, s);
+    auto row_base = [&](int i) -> size_t {
+        return jb.rect ? (size_t)(i - r0) * (size_t)jb.nB : sbase + ((size_t)i - sr0) * ((size_t)n - sr0) - sr0;
+    };
+    const int jhi = jb.rect ? jb.nB : n;
+    // one row: columns [j0, j1) of chunk ch; several rows: all of them
+    const int j0 = (jb.rect ? 0 : i0 + 1) + ch * RCW, j1 = min(jhi, j0 + RCW);
+    if (j0 >= jhi) return;
+    const size_t g0 = row_base(i0) + j0;
+    const size_t g1 = i1 - i0 == 1 ? row_base(i0) + j1 : row_base(i1 - 1) + jhi;  // cells [g0, g1)
     const size_t a0 = g0 & ~(size_t)7;              // first 16-byte word touched
     const int nq = (int)((g1 - a0 + 7) >> 3);       // 16-byte words staged
     const uint4* __restrict__ src = reinterpret_cast<const uint4*>(out + a0);
@@ -752,45 +773,50 @@ __device__ __forceinline__ void rare_rows_impl(const RareJob& jb, const M* __res
         }
     }
 
-    for (uint32_t eb = e0; eb < e1; eb += RNT) {
-        const int nb = (int)min((uint32_t)RNT, e1 - eb);
-        if (tid < nb) {
-            const uint64_t r = rentry[eb + tid];
-            uint32_t b = (uint32_t)r, end = (uint32_t)(r >> 32);
-            if (chunks > 1) {  // clip the ascending range to [j0, j1)
-                uint32_t lo = b, hi = end;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if ((int)gmem[mid] < j0) lo = mid + 1; else hi = mid;
-                }
-                b = lo;
-                hi = end;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if ((int)gmem[mid] < j1) lo = mid + 1; else hi = mid;
-                }
-                end = lo;
-            }
-            ebeg[tid] = b;
-            eend[tid] = end;
-        }
-        __syncthreads();  // also orders the staging stores before the atomics
-        const int sub = tid / RSG, sl = tid % RSG;
-        for (int e = sub; e < nb; e += RNT / RSG) {
-            const uint32_t b = ebeg[e], end = eend[e];
-            for (uint32_t p = b + sl; p < end; p += RSG * RUN) {
-                uint32_t jj[RUN];
-#pragma unroll
-                for (int u = 0; u < RUN; u++) jj[u] = p + u * RSG < end ? (uint32_t)gmem[p + u * RSG] : 0xffffffffu;
-#pragma unroll
-                for (int u = 0; u < RUN; u++)
-                    if (jj[u] != 0xffffffffu) {
-                        const size_t o = base + (size_t)jj[u] - a0;  // u16 offset in LDS
-                        atomicSub(&cells[o >> 1], (o & 1) ? 0x10000u : 1u);
+    for (int i = i0; i < i1; i++) {
+        const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
+        const size_t base = row_base(i);
+        for (uint32_t eb = e0; eb < e1; eb += RNT) {
+            const int nb = (int)min((uint32_t)RNT, e1 - eb);
+            if (tid < nb) {
+                const uint64_t r = rentry[eb + tid];
+                uint32_t b = (uint32_t)r, end = (uint32_t)(r >> 32);
+                if (chunks > 1) {  // clip the ascending range to [j0, j1)
+                    uint32_t lo = b, hi = end;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if ((int)gmem[mid] < j0) lo = mid + 1; else hi = mid;
                     }
+                    b = lo;
+                    hi = end;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if ((int)gmem[mid] < j1) lo = mid + 1; else hi = mid;
+                    }
+                    end = lo;
+                }
+                ebeg[tid] = b;
+                eend[tid] = end;
             }
+            __syncthreads();  // also orders the staging stores before the atomics
+            const int rsg = jb.rsg, sub = tid / rsg, sl = tid % rsg;
+            for (int e = sub; e < nb; e += RNT / rsg) {
+                const uint32_t b = ebeg[e], end = eend[e];
+                for (uint32_t p = b + sl; p < end; p += rsg * RUN) {
+                    uint32_t jj[RUN];
+#pragma unroll
+                    for (int u = 0; u < RUN; u++)
+                        jj[u] = p + u * rsg < end ? (uint32_t)gmem[p + u * rsg] : 0xffffffffu;
+#pragma unroll
+                    for (int u = 0; u < RUN; u++)
+                        if (jj[u] != 0xffffffffu) {
+                            const size_t o = base + (size_t)jj[u] - a0;  // u16 offset in LDS
+                            atomicSub(&cells[o >> 1], (o & 1) ? 0x10000u : 1u);
+                        }
+                }
+            }
+            __syncthreads();  // ebeg / eend are rewritten by the next batch
         }
-        __syncthreads();  // ebeg / eend are rewritten by the next batch
     }
 
     if (jb.cap > 0) {  // main.rs:251-253 on the final values
@@ -813,15 +839,28 @@ __device__ __forceinline__ void rare_rows_impl(const RareJob& jb, const M* __res
 
 // Member lists are read as u16 when every row index fits: half the bytes of
 // the pass's dominant (L2-missing) reads.
-__device__ __forceinline__ void rare_rows_body(const RareJob& jb, int i, int ch, RareShared& sh) {
-    if (jb.m16) rare_rows_impl(jb, static_cast<const uint16_t*>(jb.gmem), i, ch, sh);
-    else rare_rows_impl(jb, static_cast<const uint32_t*>(jb.gmem), i, ch, sh);
+// Workgroup b of job jb -> its rows and chunk.
+__device__ __forceinline__ void rare_rows_body(const RareJob& jb, long long b, RareShared& sh) {
+    int i0, i1, ch = 0;
+    if (jb.G > 1) {
+        const int s = (int)(b / jb.gps), g = (int)(b % jb.gps);
+        const int sr0 = jb.r0 + s * jb.S;
+        i0 = sr0 + g * jb.G;
+        i1 = min(min(i0 + jb.G, sr0 + jb.S), jb.n);
+        if (i0 >= i1) return;
+    } else {
+        i0 = jb.r0 + (int)(b / jb.chunks);
+        i1 = i0 + 1;
+        ch = (int)(b % jb.chunks);
+    }
+    if (jb.m16) rare_rows_impl(jb, static_cast<const uint16_t*>(jb.gmem), i0, i1, ch, sh);
+    else rare_rows_impl(jb, static_cast<const uint32_t*>(jb.gmem), i0, i1, ch, sh);
 }
 
-// Rows r0 .. r0 + gridDim.x / chunks - 1 of one job.
+// One job.
 __global__ void __launch_bounds__(RNT) sub_rare_rows(RareJob jb) {
     __shared__ RareShared sh;
-    rare_rows_body(jb, jb.r0 + (int)(blockIdx.x / jb.chunks), (int)(blockIdx.x % jb.chunks), sh);
+    rare_rows_body(jb, blockIdx.x, sh);
 }
 
 // Many jobs in one launch: workgroup b belongs to the last job whose
@@ -837,11 +876,27 @@ __global__ void __launch_bounds__(RNT) sub_rare_rows_batch(const RareJob* __rest
     }
     RareJob jb = jobs[lo];
     jb.out = base + outoff[jb.d];
-    const long long local = b - jb.block0;
-    rare_rows_body(jb, jb.r0 + (int)(local / jb.chunks), (int)(local % jb.chunks), sh);
+    rare_rows_body(jb, b - jb.block0, sh);
 }
 
 size_t up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// Workgroup layout of a rare job over `rows` rows from jb.r0: row groups
+// when G >= 2 rows of the widest stripe fit one chunk, else one row per
+// workgroup in column chunks.  Returns the number of workgroups.
+long long rare_layout(RareJob& jb, int rows) {
+    const int pitch = jb.rect ? jb.nB : jb.n - jb.r0;  // the widest row of the layout
+    jb.G = jb.rect ? 1 : std::max(1, RCW / std::max(pitch, 1));
+    if (jb.G > 1) {
+        jb.chunks = 1;
+        jb.gps = (jb.S + jb.G - 1) / jb.G;
+        return (long long)((rows + jb.S - 1) / jb.S) * jb.gps;
+    }
+    const int width = jb.rect ? jb.nB : jb.n - jb.r0 - 1;
+    jb.chunks = std::max(1, (width + RCW - 1) / RCW);
+    jb.gps = 0;
+    return (long long)rows * jb.chunks;
+}
 
 void set_err(char* err, int errlen, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 void set_err(char* err, int errlen, const char* fmt, ...) {
@@ -1420,9 +1475,10 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
         HK(launch_tiles(ds_kernel(ds, kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
                          S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
-        const int chunks = (n - 1 + RCW - 1) / RCW;
-        const RareJob jb = {d_out, n, 0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16, ds->cap};
-        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)n * chunks)), dim3(RNT), 0, st, jb);
+        RareJob jb = {d_out, n, 0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16, ds->cap, 1, 0,
+                      lanes_per_range(ds->n_rare, ds->n_rentries)};
+        const long long blocks = rare_layout(jb, n);
+        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)blocks), dim3(RNT), 0, st, jb);
         HK(hipGetLastError());
     }
     return 0;
@@ -1513,10 +1569,10 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
                 for (int J = 0; J * tl < w; J++) lists[k].push_back({J >= I ? cost : 0, make_int4(d, s, I, J)});
         }
         if (ds->n_rare && ds->n > 1) {
-            RareJob r = {NULL, ds->n, 0, S, (ds->n - 1 + RCW - 1) / RCW, ds->drstart, ds->drentry, ds->dgmem,
-                         b->rare_blocks, d, 0, 0, ds->m16, ds->cap};
+            RareJob r = {NULL, ds->n, 0, S, 1, ds->drstart, ds->drentry, ds->dgmem, b->rare_blocks, d, 0, 0, ds->m16,
+                         ds->cap, 1, 0, lanes_per_range(ds->n_rare, ds->n_rentries)};
+            b->rare_blocks += rare_layout(r, ds->n);
             rjobs.push_back(r);
-            b->rare_blocks += (long long)ds->n * r.chunks;
         }
     }
     b->nrare = (int)rjobs.size();
@@ -1883,9 +1939,9 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     else if (fp4) hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, sl.st, a);
     else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);
     if (ne) {
-        const int chunks = (db->n + RCW - 1) / RCW;
-        const RareJob jb = {sl.ddist, q, 0, q, chunks, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n, db->m16, 0};
-        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)q * chunks)), dim3(RNT), 0, sl.st, jb);
+        RareJob jb = {sl.ddist, q, 0, q, 1, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n, db->m16, 0, 1, 0, 32};
+        const long long blocks = rare_layout(jb, q);
+        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)blocks), dim3(RNT), 0, sl.st, jb);
     }
     const int tchunks = (db->n + TKC - 1) / TKC;
     hipLaunchKernelGGL(top_k_part, dim3((unsigned)((size_t)q * tchunks)), dim3(256), 0, sl.st, sl.ddist, db->n,
@@ -2168,10 +2224,10 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
             HK(launch_tiles(ds_kernel(ds, CGH_KERNEL_AUTO), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2,
                             (int)ds->n_pad, n, r0, S, 1, dU[b], st[b], ds->drstart, ds->cap));
             if (ds->n_rare && n - r0 > 1) {
-                const int chunks = (n - r0 - 1 + RCW - 1) / RCW;
-                const RareJob jb = {dU[b], n, r0, S, chunks, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0,
-                                    ds->m16, ds->cap};
-                hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)((size_t)rows * chunks)), dim3(RNT), 0, st[b], jb);
+                RareJob jb = {dU[b], n, r0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16, ds->cap, 1,
+                              0, lanes_per_range(ds->n_rare, ds->n_rentries)};
+                const long long blocks = rare_layout(jb, rows);
+                hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)blocks), dim3(RNT), 0, st[b], jb);
                 HK(hipGetLastError());
             }
             HK(hipMemcpyAsync(hU[b], dU[b], (size_t)rows * w * 2, hipMemcpyDeviceToHost, st[b]));
