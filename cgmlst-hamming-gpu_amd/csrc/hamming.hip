@@ -142,24 +142,31 @@ __device__ __forceinline__ Acc mfma_i8(v4i a, v4i b, Acc c) {
     else return c;
 }
 
-// fp4 (e2m1) operands: 0 -> 0x0, +1.0 -> 0x2, -1.0 -> 0xA.  A 32-bit word of
-// one row becomes the 32 nibbles of a lane's 64-deep operand half: nibble q
-// of dword t is bit 4q + t.  Every product is 0 or +-1 and every partial sum
-// an integer of magnitude <= L <= 65,535, so the fp32 accumulation is exact.
-__device__ __forceinline__ v8i expand_fp4(uint32_t w) {
+// fp4 (e2m1) operands.  A 32-bit word of one row becomes the 32 nibbles of
+// a lane's 64-deep operand half: nibble q of dword t holds bit 4q + t.  The
+// nibble values differ by dword so that the A side needs no shift for
+// t = 0, 1, 2 (the bit stays in place: 0x1 = 0.5, 0x2 = 1.0, 0x4 = 2.0) and
+// the B side carries the reciprocals (2.0, 1.0, 0.5); t = 3 is 1.0 on both
+// sides (bit 3 would be the sign).  Every product is 0 or exactly 1 and every
+// partial sum an integer of magnitude <= L <= 65,535, so the fp32
+// accumulation is exact.
+__device__ __forceinline__ v8i expand_fp4_a(uint32_t w) {
     v8i r;
-    r[0] = (int)((w << 1) & 0x22222222u);
+    r[0] = (int)(w & 0x11111111u);
     r[1] = (int)(w & 0x22222222u);
-    r[2] = (int)((w >> 1) & 0x22222222u);
+    r[2] = (int)(w & 0x44444444u);
     r[3] = (int)((w >> 2) & 0x22222222u);
     r[4] = r[5] = r[6] = r[7] = 0;
     return r;
 }
 
-__device__ __forceinline__ v8i expand_fp4_neg(uint32_t w) {
-    v8i r = expand_fp4(w);
-#pragma unroll
-    for (int t = 0; t < 4; t++) r[t] |= r[t] << 2;  // 0x2 -> 0xA
+__device__ __forceinline__ v8i expand_fp4_b(uint32_t w) {
+    v8i r;
+    r[0] = (int)((w << 2) & 0x44444444u);
+    r[1] = (int)(w & 0x22222222u);
+    r[2] = (int)((w >> 2) & 0x11111111u);
+    r[3] = (int)((w >> 2) & 0x22222222u);
+    r[4] = r[5] = r[6] = r[7] = 0;
     return r;
 }
 
@@ -266,8 +273,9 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
 #pragma unroll
     for (int p = 0; p < NST - 1; p++) issue_slab(min(p, nc - 1), p);
 
-    // acc accumulates O O^T - P P^T: O slabs with B = +O, P slabs with
-    // B = -P (the P slabs are the short ones), negated in the epilogue.
+    // int8: acc accumulates O O^T - P P^T (O slabs with B = +O, P slabs with
+    // B = -P, the short ones), negated in the epilogue.  fp4: acc holds
+    // O O^T, is negated once between the phases and then adds P P^T.
     // Words of group gi+1 are read from LDS while group gi is multiplied.
     // fp4: each lane half g reads words 2g, 2g+1 of a 4-word group (one
     // 8-byte read), one word per 64-deep MFMA step.
@@ -293,12 +301,9 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
             for (int kk = 0; kk < 2; kk++) {
                 v8i av[TM], bv[TN];
 #pragma unroll
-                for (int i = 0; i < TM; i++) av[i] = expand_fp4(kk ? wa[i].y : wa[i].x);
+                for (int i = 0; i < TM; i++) av[i] = expand_fp4_a(kk ? wa[i].y : wa[i].x);
 #pragma unroll
-                for (int j = 0; j < TN; j++) {
-                    const uint32_t x = kk ? wb[j].y : wb[j].x;
-                    bv[j] = decltype(neg)::value ? expand_fp4_neg(x) : expand_fp4(x);
-                }
+                for (int j = 0; j < TN; j++) bv[j] = expand_fp4_b(kk ? wb[j].y : wb[j].x);
 #pragma unroll
                 for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -370,12 +375,22 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     };
     int c = 0;
     for (; c < nc1; c++) step(c, std::false_type{});
-    for (; c < nc; c++) step(c, std::true_type{});
+    if constexpr (FP4) {
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[i][j][e] = -acc[i][j][e];
+        for (; c < nc; c++) step(c, std::false_type{});
+    } else {
+        for (; c < nc; c++) step(c, std::true_type{});
+    }
     // the trailing reloads must land before the workgroup's LDS is released
     __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
 
-    // Epilogue: lane holds column l32 and rows 8*(e/4) + 4*g + (e%4) of
-    // O O^T - P P^T; the distance is its negation.  The capped form is a
+    // Epilogue: lane holds column l32 and rows 8*(e/4) + 4*g + (e%4) of the
+    // distance (fp4) or of its negation (int8).  The capped form is a
     // separate loop so that the common one stays a plain store sequence.
     auto store_all = [&](auto capped) {
 #pragma unroll
@@ -388,7 +403,7 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
                 for (int e = 0; e < 16; e++) {
                     const int r = rt0 + wm0 + i * 32 + (e >> 2) * 8 + g * 4 + (e & 3);
                     if (r >= rows) continue;
-                    const uint32_t v = (uint32_t)(-(int)acc[i][j][e]);
+                    const uint32_t v = FP4 ? (uint32_t)(int)acc[i][j][e] : (uint32_t)(-(int)acc[i][j][e]);
                     uint16_t o;
                     if (a.rect) o = (uint16_t)v;
                     else if constexpr (decltype(capped)::value) o = c > r ? epilogue_cap(a, sr0 + r, v) : 0;
