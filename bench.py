@@ -25,7 +25,8 @@ N, L, S = 63005, 1142, 4096
 # fp32 rate (78.6 T lane-instructions/s), one AND + one v_bcnt per 32 bit
 # pairs = 32 ops per lane-instruction.
 PEAKS = {1: (5000.0, "TOP/s", "int8 (0/1 operands, int32 accumulate)"),
-         2: (10100.0, "TFLOP/s", "fp4 e2m1 (0/+-1 operands, fp32 accumulate of integers <= L: exact)"),
+         2: (10100.0, "TFLOP/s", "fp4 e2m1 (bits as 0 or 0.5/1/2 against reciprocal partners: every product 0 or 1; "
+                                 "fp32 accumulate of integers <= L: exact)"),
          3: (2517.0, "TOP/s", "u32 (VALU AND + popcount, u32 accumulate)")}
 MEASURED_PEAK_KEY = {1: "int8_mfma_tops", 2: "fp4_mfma_tflops", 3: "valu_popcount_tops"}
 
