@@ -596,14 +596,14 @@ int main(int argc, char** argv) {
     int eno = 0;
     if (!read_all(args.input, &text, &eno)) die("reading " + args.input + ": " + io_error(eno));
     if (!valid_utf8(text)) die("reading " + args.input + ": stream did not contain valid UTF-8");
-    // Rust str::lines(): split on '\n', drop one trailing '\r' per line, no
-    // final empty line after a trailing '\n'.
+    // Rust str::lines(): split on '\n', drop the '\r' of a "\r\n", no final
+    // empty line after a trailing '\n'; a bare '\r' ending the text stays.
     std::vector<std::pair<size_t, size_t>> lines;
     for (size_t p = 0; p < text.size();) {
         size_t q = text.find('\n', p);
         const size_t end = q == std::string::npos ? text.size() : q;
         size_t e = end;
-        if (e > p && text[e - 1] == '\r') e--;
+        if (q != std::string::npos && e > p && text[e - 1] == '\r') e--;
         lines.emplace_back(p, e);
         p = q == std::string::npos ? text.size() : q + 1;
     }

@@ -20,6 +20,12 @@ Contents and the reference lines each restates:
                           rare groups, CSR) in numpy, the golden of the
                           encoder.
   relu_edge_vector        python/relu_hamming.py:56-57 known answer.
+  rust_lines, parse_allele, load_tsv, completeness_filters, render_matrix
+                          main.rs:95-113 (cell semantics), 286-311 (load:
+                          Rust str::lines, header width, empty lines dropped,
+                          rows padded / truncated), 313-347 (-L then -S in
+                          f64), 225-265 (text of the matrix), the expected
+                          bytes of the CLI edge tests.
 
 Pins: see tests/test_oracle.py (checksum KAT 85724a9d7d090f9f of
 tensorcore/README.md:52, the edge vector, hand counts, invariants, and the
@@ -272,3 +278,110 @@ def relu_edge_vector():
     a = np.array([5, 5, 6, 0, 7, 0, 10**9])
     b = np.array([5, 6, 5, 3, 0, 0, 1])
     return a, b, np.array([0, 1, 1, 0, 0, 0, 1])
+
+
+# ---------------------------------------------------------------------------
+# Input semantics of the command line (main.rs:95-113, 286-347)
+# ---------------------------------------------------------------------------
+
+class AlleleTooLarge(ValueError):
+    pass
+
+
+def rust_lines(text):
+    """str::lines as main.rs:287 uses it: split at LF, drop the CR of a CRLF,
+    no final empty line after a trailing LF; a bare CR ending the text is
+    kept (documented behaviour of str::lines since Rust 1.77)."""
+    if text == "":
+        return []
+    parts = text.split("\n")
+    ended = text.endswith("\n")
+    if ended:
+        parts = parts[:-1]
+    last = len(parts) - 1
+    return [p[:-1] if p.endswith("\r") and (i < last or ended) else p
+            for i, p in enumerate(parts)]
+
+
+def parse_allele(cell, skip_repl=False, missing="-"):
+    """main.rs:95-113: trim, strip one leading INF- (unless skip_repl), the
+    missing value or empty -> 0, all ASCII digits -> u64 (overflow raises),
+    anything else -> 0."""
+    s = cell.strip()
+    if not skip_repl and s.startswith("INF-"):
+        s = s[4:]
+    if s == missing or s == "":
+        return 0
+    if all("0" <= ch <= "9" for ch in s):
+        v = int(s)
+        if v >= 1 << 64:
+            raise AlleleTooLarge(s)
+        return v
+    return 0
+
+
+def load_tsv(text, sep="\t", skip_repl=False, missing="-"):
+    """main.rs:286-311 -> (ids, calls n x L u64).  L = header fields - 1;
+    empty lines are dropped; every cell of a row is parsed, then the row is
+    padded with 0 or cut to L."""
+    lines = rust_lines(text)
+    if not lines:
+        raise ValueError("empty input")
+    L = len(lines[0].split(sep)) - 1
+    rows = [ln for ln in lines[1:] if ln != ""]
+    ids, calls = [], np.zeros((len(rows), L), dtype=np.uint64)
+    for i, ln in enumerate(rows):
+        f = ln.split(sep)
+        ids.append(f[0])
+        vals = [parse_allele(c, skip_repl, missing) for c in f[1:]]
+        k = min(L, len(vals))
+        calls[i, :k] = np.array(vals[:k], dtype=np.uint64)
+    return ids, calls
+
+
+def completeness_filters(ids, calls, locus_pct=None, sample_pct=None):
+    """main.rs:313-347: keep loci with (called / n) * 100 >= thr, then samples
+    with (called / L) * 100 >= thr, in f64 and in that order."""
+    n, L = calls.shape
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if locus_pct is not None:
+            keep = [k for k in range(L)
+                    if (np.float64(np.count_nonzero(calls[:, k])) / np.float64(n)) * 100.0 >= locus_pct]
+            calls = calls[:, keep]
+            L = calls.shape[1]
+        if sample_pct is not None:
+            keep = [i for i in range(n)
+                    if (np.float64(np.count_nonzero(calls[i])) / np.float64(L)) * 100.0 >= sample_pct]
+            calls = calls[keep]
+            ids = [ids[i] for i in keep]
+    return ids, calls
+
+
+def render_matrix(ids, H, mode="full", cap=0, osep="\t", index_name="cgmlst-hamming-gpu"):
+    """main.rs:225-265: header index_name + (osep id)*, then per row the id and
+    every cell; cells outside the layout print 0, the cap applies when
+    cap > 0 and v > cap; LF line ends."""
+    n = len(ids)
+    out = [index_name + "".join(osep + x for x in ids) + "\n"]
+    for i in range(n):
+        row = [ids[i]]
+        for j in range(n):
+            keep = mode == "full" or (mode == "lower-tri" and j <= i) or (mode == "upper-tri" and j >= i)
+            v = int(H[i, j]) if keep else 0
+            if cap > 0 and v > cap:
+                v = cap
+            row.append(str(v))
+        out.append(osep.join(row) + "\n")
+    return "".join(out)
+
+
+def cli_expected(text, sep="\t", skip_repl=False, missing="-", locus_pct=None, sample_pct=None, **fmt):
+    """Expected stdout of the command line for an input text, or raises the
+    error main.rs dies with (AlleleTooLarge, ValueError)."""
+    ids, calls = load_tsv(text, sep, skip_repl, missing)
+    ids, calls = completeness_filters(ids, calls, locus_pct, sample_pct)
+    if calls.shape[1] > 65535:
+        raise ValueError("more than 65535 loci (u16 distances)")
+    n, L = calls.shape
+    H = brute_full(dense_ranks(calls)) if n and L else np.zeros((n, n), dtype=np.int64)
+    return render_matrix(ids, H, **fmt)

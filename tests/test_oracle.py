@@ -118,3 +118,98 @@ def test_checksum_kat_synthetic_30k():
     A = oracle.gen_profiles(50000, 5000)[:30000]
     X = oracle.ranks_checkpy_style(A)
     assert oracle.checksum_upper(X) == 0x85724a9d7d090f9f
+
+
+# ------------------------------------------------ command-line input / output
+
+def test_rust_lines_documented_examples():
+    # the examples of Rust's str::lines documentation
+    assert oracle.rust_lines("foo\r\nbar\n\nbaz\n") == ["foo", "bar", "", "baz"]
+    assert oracle.rust_lines("foo\nbar\n\r\nbaz") == ["foo", "bar", "", "baz"]
+    assert oracle.rust_lines("foo\nbar\n\r\nbaz\r") == ["foo", "bar", "", "baz\r"]
+    assert oracle.rust_lines("") == []
+    assert oracle.rust_lines("\n") == [""]
+
+
+@pytest.mark.parametrize("case", oracle.CHECK_CASES)
+def test_load_tsv_recovers_check_py_ids(case):
+    """check.py:17-36 writes '-', '', 'LNF', 'PLOT3' for missing and 'INF-v'
+    or 'v' for allele v, and expects distances of A (check.py:39-44): the
+    load must give A itself (README.md:81: '-', empty and non-numeric calls
+    are missing, INF- is stripped), with LF and CRLF line ends."""
+    n, L, seed, hashed = case
+    A, cells = oracle.check_make(n, L, seed, hashed)
+    text = "FILE\t" + "\t".join(f"L{j}" for j in range(L)) + "\n"
+    text += "".join(f"S{i}\t" + "\t".join(row) + "\n" for i, row in enumerate(cells))
+    for t in (text, text.replace("\n", "\r\n")):
+        ids, calls = oracle.load_tsv(t)
+        assert ids == [f"S{i}" for i in range(n)]
+        assert np.array_equal(calls, A.astype(np.uint64))
+    # -r keeps INF-v as a non-numeric call: exactly those cells become missing
+    _, calls_r = oracle.load_tsv(text, skip_repl=True)
+    inf = np.array([[c.startswith("INF-") for c in row] for row in cells])
+    assert inf.any()
+    assert np.array_equal(calls_r, np.where(inf, 0, A).astype(np.uint64))
+
+
+def test_parse_allele_u64_range():
+    assert oracle.parse_allele(str(2**64 - 1)) == 2**64 - 1
+    assert oracle.parse_allele("007") == 7
+    with pytest.raises(oracle.AlleleTooLarge):
+        oracle.parse_allele(str(2**64))
+    with pytest.raises(oracle.AlleleTooLarge):
+        oracle.parse_allele("INF-" + "9" * 25)
+
+
+def test_rows_padded_truncated_and_empty_lines():
+    text = "id\tA\tB\tC\n\nx\t1\ny\t1\t2\t3\t4\t5\n\nz\n"
+    ids, calls = oracle.load_tsv(text)
+    assert ids == ["x", "y", "z"]
+    assert calls.tolist() == [[1, 0, 0], [1, 2, 3], [0, 0, 0]]
+    # a cell past the header width is still parsed: its overflow is an error
+    with pytest.raises(oracle.AlleleTooLarge):
+        oracle.load_tsv("id\tA\nx\t1\t" + "9" * 21 + "\n")
+
+
+def test_completeness_filters_f64_and_order():
+    from fractions import Fraction
+    # (57 / 100) * 100 is 56.99999999999999 in binary64: a locus called in 57
+    # of 100 samples fails -L 57, although 57% are called
+    calls = np.zeros((100, 2), dtype=np.uint64)
+    calls[:57, 0] = 3
+    calls[:, 1] = 4
+    assert (57 / 100) * 100 < 57
+    _, c = oracle.completeness_filters([str(i) for i in range(100)], calls, locus_pct=57)
+    assert c.shape == (100, 1)
+    _, c = oracle.completeness_filters([str(i) for i in range(100)], calls, locus_pct=56.9)
+    assert c.shape == (100, 2)
+    # away from binary64 rounding the rule is the exact rational one
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        n, L = rng.integers(1, 30, 2)
+        X = (rng.random((n, L)) < rng.random()).astype(np.uint64)
+        thr = float(rng.integers(0, 101)) + 0.5
+        ids, Y = oracle.completeness_filters(list(range(n)), X, locus_pct=thr)
+        keep = [k for k in range(L) if Fraction(int(X[:, k].sum()) * 100, int(n)) >= Fraction(thr)]
+        assert np.array_equal(Y, X[:, keep])
+    # the locus filter runs first, the sample filter then sees only kept loci
+    X = np.array([[1, 0, 0], [1, 0, 0], [1, 5, 0]], dtype=np.uint64)
+    ids, Y = oracle.completeness_filters(["a", "b", "c"], X, locus_pct=50, sample_pct=100)
+    assert ids == ["a", "b", "c"] and Y.shape == (3, 1)
+    ids, Y = oracle.completeness_filters(["a", "b", "c"], X, sample_pct=50)
+    assert ids == ["c"] and Y.shape == (1, 3)
+    # no samples: 0 / 0 is NaN, every locus is dropped
+    ids, Y = oracle.completeness_filters([], np.zeros((0, 4), dtype=np.uint64), locus_pct=0)
+    assert Y.shape == (0, 0)
+
+
+def test_render_matrix_known_text():
+    H = np.array([[0, 3, 9], [3, 0, 1], [9, 1, 0]])
+    ids = ["a", "b", "c"]
+    assert oracle.render_matrix(ids, H) == \
+        "cgmlst-hamming-gpu\ta\tb\tc\na\t0\t3\t9\nb\t3\t0\t1\nc\t9\t1\t0\n"
+    assert oracle.render_matrix(ids, H, mode="lower-tri", cap=4, osep="::", index_name="x") == \
+        "x::a::b::c\na::0::0::0\nb::3::0::0\nc::4::1::0\n"
+    assert oracle.render_matrix(ids, H, mode="upper-tri") == \
+        "cgmlst-hamming-gpu\ta\tb\tc\na\t0\t3\t9\nb\t0\t0\t1\nc\t0\t0\t0\n"
+    assert oracle.render_matrix([], np.zeros((0, 0))) == "cgmlst-hamming-gpu\n"
