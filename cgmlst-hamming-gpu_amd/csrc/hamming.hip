@@ -35,6 +35,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -42,6 +43,7 @@
 #include <string>
 #include <thread>
 #include <type_traits>
+#include <utility>
 #include <unordered_map>
 #include <vector>
 
@@ -695,26 +697,24 @@ sub_rare_csr(uint16_t* out, int w, int r0, int rows, const uint32_t* start, cons
     for (uint32_t p = start[i] + lane; p < start[i + 1]; p += 64) sub_one_u16(out, rowoff + js[p]);
 }
 
-// Rare pairs, row-local: one workgroup per (row i, chunk of RCW columns).
-// The row's cells j in (i, n) of the chunk are staged in LDS with 16-byte
-// loads, every rare partner j of row i is subtracted there with an LDS
-// atomic, and the chunk is written back: HBM sees each output cell read and
-// written once and the scattered updates stay on-chip.  Row i's partners are
-// ranges of the ascending group member lists (rentry).  A batch of up to RNT
-// ranges is read (and clipped to the chunk) by all threads at once into LDS;
-// then each 32-lane subgroup takes ranges from LDS, its lanes reading
-// consecutive members (RUN loads in flight per lane) and applying them.
+// Rare pairs, row-local: a workgroup takes items (row i and a chunk of RCW
+// columns, or a group of short rows) one after the other.  An item's cells
+// are staged in LDS with 16-byte loads, every rare partner j of its rows is
+// subtracted there with an LDS atomic, and the cells are written back: HBM
+// sees each output cell read and written once and the scattered updates stay
+// on-chip.  Row i's partners are ranges of the ascending group member lists
+// (rentry).
 constexpr int RCW = 65536;  // u16 cells per chunk (128 KiB of LDS: one workgroup per CU)
 constexpr int RNT = 1024;   // threads per workgroup = ranges per batch
 constexpr int RQ = (RCW / 8 + 1 + RNT - 1) / RNT;  // 16-byte words staged per thread
 constexpr int RUN = 8;      // partners per lane per pass
 // Lanes per range: about 0.4 x the mean range length (partners per range),
-// a power of two in [4, 64]: short ranges leave fewer lanes idle, long ones
+// a power of two in [4, 16]: short ranges leave fewer lanes idle, long ones
 // keep fewer ranges in flight.
 int lanes_per_range(uint64_t pairs, size_t ranges) {
     const double mean = ranges ? (double)pairs / (double)ranges : 1.0;
     int g = 4;
-    while (g < 64 && g * 2 <= mean * 0.4) g *= 2;
+    while (g < 16 && g * 2 <= mean * 0.4) g *= 2;
     return g;
 }
 struct RareJob {
@@ -732,154 +732,237 @@ struct RareJob {
     // short enough that G of them fit one chunk), gps groups per stripe;
     // G = 1: one row per workgroup, in `chunks` column chunks.
     int G, gps;
-    int rsg;                   // lanes per range (a power of two, 4 .. 64)
+    int rsg;                   // lanes per range (a power of two, 4 .. 32)
 };
 
 struct RareShared {
     uint4 cells4[RCW / 8 + 1];
     uint32_t ebeg[RNT];  // batch ranges [ebeg, eend) of member indices
     uint32_t eend[RNT];
+    uint32_t spare[64];  // one word per lane for the atomics of idle slots
+    uint32_t next;       // next range of the batch to take
 };
 
-// Rows i0 .. i1-1 (>= r0, one stripe) of job jb, members of type M: with
-// one row, column chunk ch of it; with several, every cell j > i of each row
-// (they fit one chunk).  The cells are staged as one contiguous range of the
-// layout (for several rows it includes their cells j <= i, which no partner
-// touches and which are written back unchanged).
-template <typename M>
-__device__ __forceinline__ void rare_rows_impl(const RareJob& jb, const M* __restrict__ gmem, int i0, int i1, int ch,
-                                               RareShared& sh) {
-    uint16_t* __restrict__ out = jb.out;
-    const int n = jb.n, r0 = jb.r0, S = jb.S, chunks = jb.chunks;
-    const uint32_t* __restrict__ rstart = jb.rstart;
-    const uint64_t* __restrict__ rentry = jb.rentry;
-    uint4* cells4 = sh.cells4;
-    uint32_t* ebeg = sh.ebeg;
-    uint32_t* eend = sh.eend;
-    uint32_t* cells = reinterpret_cast<uint32_t*>(cells4);
-    const int tid = threadIdx.x;
-    if (rstart[i0] == rstart[i1]) return;  // no rare partners in these rows
+// Work item b of job jb: with row groups (G > 1) rows i0 .. i1-1 of one
+// stripe, every cell j > i of each (they fit one chunk); else one row i0 and
+// column chunk ch of it.  The cells are staged as one contiguous range
+// [g0, g1) of the layout (for several rows it includes their cells j <= i,
+// which no partner touches and which are written back unchanged), as nq
+// 16-byte words from a0.  False when the item has no rare partners.
+struct RareItem {
+    int i0, i1, j0, j1;
+    size_t sbase, sr0, g0, g1, a0;
+    int nq;
+};
+
+__device__ __forceinline__ bool rare_item(const RareJob& jb, long long b, RareItem& it) {
+    int ch = 0;
+    if (jb.G > 1) {
+        const int s = (int)(b / jb.gps), g = (int)(b % jb.gps);
+        const int sr0 = jb.r0 + s * jb.S;
+        it.i0 = sr0 + g * jb.G;
+        it.i1 = min(min(it.i0 + jb.G, sr0 + jb.S), jb.n);
+        if (it.i0 >= it.i1) return false;
+    } else {
+        it.i0 = jb.r0 + (int)(b / jb.chunks);
+        it.i1 = it.i0 + 1;
+        ch = (int)(b % jb.chunks);
+    }
+    if (jb.rstart[it.i0] == jb.rstart[it.i1]) return false;  // no rare partners in these rows
     // symmetric: cells j in (i, n) of the stripe layout of rows r0..
     // (stripe s = rows r0+s*S ..); rectangular: cells j in [0, nB) of row
-    // i - r0 of an (n - r0) x nB row-major matrix.  u16 index of (i, j) is
-    // row_base(i) + j.
-    const size_t s = jb.rect ? 0 : (size_t)((i0 - r0) / S), sr0 = (size_t)r0 + s * S;
-    const size_t sbase = jb.rect ? 0 : stripe_off((size_t)(n - r0), (size_t)S, s);
+    // i - r0 of an (n - r0) x nB row-major matrix.
+    const size_t s = jb.rect ? 0 : (size_t)((it.i0 - jb.r0) / jb.S);
+    it.sr0 = (size_t)jb.r0 + s * jb.S;
+    it.sbase = jb.rect ? 0 : stripe_off((size_t)(jb.n - jb.r0), (size_t)jb.S, s);
+    const int jhi = jb.rect ? jb.nB : jb.n;
+    it.j0 = (jb.rect ? 0 : it.i0 + 1) + ch * RCW;
+    it.j1 = min(jhi, it.j0 + RCW);
+    if (it.j0 >= jhi) return false;
     auto row_base = [&](int i) -> size_t {
-        return jb.rect ? (size_t)(i - r0) * (size_t)jb.nB : sbase + ((size_t)i - sr0) * ((size_t)n - sr0) - sr0;
+        return jb.rect ? (size_t)(i - jb.r0) * (size_t)jb.nB
+                       : it.sbase + ((size_t)i - it.sr0) * ((size_t)jb.n - it.sr0) - it.sr0;
     };
-    const int jhi = jb.rect ? jb.nB : n;
-    // one row: columns [j0, j1) of chunk ch; several rows: all of them
-    const int j0 = (jb.rect ? 0 : i0 + 1) + ch * RCW, j1 = min(jhi, j0 + RCW);
-    if (j0 >= jhi) return;
-    const size_t g0 = row_base(i0) + j0;
-    const size_t g1 = i1 - i0 == 1 ? row_base(i0) + j1 : row_base(i1 - 1) + jhi;  // cells [g0, g1)
-    const size_t a0 = g0 & ~(size_t)7;              // first 16-byte word touched
-    const int nq = (int)((g1 - a0 + 7) >> 3);       // 16-byte words staged
-    const uint4* __restrict__ src = reinterpret_cast<const uint4*>(out + a0);
-    {  // every staging load in flight before the first LDS write
-        uint4 v[RQ];
-#pragma unroll
-        for (int u = 0; u < RQ; u++) v[u] = src[min(tid + u * RNT, nq - 1)];  // defined: v stays in registers
-#pragma unroll
-        for (int u = 0; u < RQ; u++) {
-            const int k = tid + u * RNT;
-            if (k < nq) cells4[k] = v[u];
-        }
-    }
+    it.g0 = row_base(it.i0) + it.j0;
+    it.g1 = it.i1 - it.i0 == 1 ? row_base(it.i0) + it.j1 : row_base(it.i1 - 1) + jhi;
+    it.a0 = it.g0 & ~(size_t)7;                    // first 16-byte word touched
+    it.nq = (int)((it.g1 - it.a0 + 7) >> 3);       // 16-byte words staged
+    return true;
+}
 
-    for (int i = i0; i < i1; i++) {
+// u16 index of cell (i, j) is rare_row_base(i) + j.
+__device__ __forceinline__ size_t rare_row_base(const RareJob& jb, const RareItem& it, int i) {
+    return jb.rect ? (size_t)(i - jb.r0) * (size_t)jb.nB
+                   : it.sbase + ((size_t)i - it.sr0) * ((size_t)jb.n - it.sr0) - it.sr0;
+}
+
+// Staging loads of an item's words into registers (thread t holds words
+// t, t + RNT, ...; indexes past nq reload word nq-1).  The indexes into v are
+// template constants from the start, so v never becomes a stack array.
+template <size_t... U>
+__device__ __forceinline__ void rare_stage_issue(const RareJob& jb, const RareItem& it, v4i (&v)[RQ],
+                                                 std::index_sequence<U...>) {
+    const v4i* __restrict__ src = reinterpret_cast<const v4i*>(jb.out + it.a0);
+    const int t = threadIdx.x, last = it.nq - 1;
+    ((v[U] = src[min(t + (int)U * RNT, last)]), ...);
+}
+
+template <size_t... U>
+__device__ __forceinline__ void rare_stage_land(const RareItem& it, const v4i (&v)[RQ], RareShared& sh,
+                                                std::index_sequence<U...>) {
+    v4i* cells = reinterpret_cast<v4i*>(sh.cells4);
+    const int t = threadIdx.x;
+    ((t + (int)U * RNT < it.nq ? (void)(cells[t + U * RNT] = v[U]) : (void)0), ...);
+}
+
+// Every rare partner of the item's rows is subtracted from its staged cell.
+// Row i's partners are ranges of the ascending group member lists (rentry):
+// a batch of up to RNT ranges is read (and clipped to the chunk) by all
+// threads at once into LDS; then each rsg-lane subgroup takes ranges from
+// LDS, its lanes reading consecutive members (RUN loads in flight per lane)
+// and applying them with LDS atomics.  The first barrier also orders the
+// staging stores before the atomics.
+template <typename M>
+__device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restrict__ gmem, const RareItem& it,
+                                            RareShared& sh) {
+    const uint32_t* __restrict__ rstart = jb.rstart;
+    const uint64_t* __restrict__ rentry = jb.rentry;
+    uint32_t* ebeg = sh.ebeg;
+    uint32_t* eend = sh.eend;
+    uint32_t* cells = reinterpret_cast<uint32_t*>(sh.cells4);
+    const int tid = threadIdx.x;
+    for (int i = it.i0; i < it.i1; i++) {
         const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
-        const size_t base = row_base(i);
+        const size_t base = rare_row_base(jb, it, i);
         for (uint32_t eb = e0; eb < e1; eb += RNT) {
             const int nb = (int)min((uint32_t)RNT, e1 - eb);
             if (tid < nb) {
                 const uint64_t r = rentry[eb + tid];
                 uint32_t b = (uint32_t)r, end = (uint32_t)(r >> 32);
-                if (chunks > 1) {  // clip the ascending range to [j0, j1)
+                if (jb.chunks > 1) {  // clip the ascending range to [j0, j1)
                     uint32_t lo = b, hi = end;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) >> 1;
-                        if ((int)gmem[mid] < j0) lo = mid + 1; else hi = mid;
+                        if ((int)gmem[mid] < it.j0) lo = mid + 1; else hi = mid;
                     }
                     b = lo;
                     hi = end;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) >> 1;
-                        if ((int)gmem[mid] < j1) lo = mid + 1; else hi = mid;
+                        if ((int)gmem[mid] < it.j1) lo = mid + 1; else hi = mid;
                     }
                     end = lo;
                 }
                 ebeg[tid] = b;
                 eend[tid] = end;
             }
-            __syncthreads();  // also orders the staging stores before the atomics
-            const int rsg = jb.rsg, sub = tid / rsg, sl = tid % rsg;
-            for (int e = sub; e < nb; e += RNT / rsg) {
+            if (tid == 0) sh.next = RNT / jb.rsg;  // ranges below this are taken without asking
+            __syncthreads();
+            // Subgroup sub starts with range sub.  With long ranges (16 lanes
+            // or more each) it then takes the next free one from sh.next, so a
+            // few long ranges do not leave the other subgroups idle at the
+            // barrier; short ones go round-robin (a take per range would cost
+            // more than the imbalance).  Loads and atomics are issued for every
+            // slot: past the range end a lane reloads the last member and
+            // adds 0 to a word of its own (no branches, no waits on them).
+            const int rsg = jb.rsg, sub = tid / rsg, sl = tid % rsg, leader = (tid & 63) - sl;
+            const uint32_t own = (uint32_t)(reinterpret_cast<uint32_t*>(sh.spare) - cells) + (tid & 63);
+            const uint32_t bo = (uint32_t)(base - it.a0);  // LDS u16 offset of cell (i, 0)
+            for (int e = sub; e < nb;) {
                 const uint32_t b = ebeg[e], end = eend[e];
-                for (uint32_t p = b + sl; p < end; p += rsg * RUN) {
+                for (uint32_t p0 = b; p0 < end; p0 += rsg * RUN) {
+                    const uint32_t p = p0 + sl;
                     uint32_t jj[RUN];
 #pragma unroll
-                    for (int u = 0; u < RUN; u++)
-                        jj[u] = p + u * rsg < end ? (uint32_t)gmem[p + u * rsg] : 0xffffffffu;
+                    for (int u = 0; u < RUN; u++) jj[u] = (uint32_t)gmem[min(p + u * rsg, end - 1)];
 #pragma unroll
-                    for (int u = 0; u < RUN; u++)
-                        if (jj[u] != 0xffffffffu) {
-                            const size_t o = base + (size_t)jj[u] - a0;  // u16 offset in LDS
-                            atomicSub(&cells[o >> 1], (o & 1) ? 0x10000u : 1u);
-                        }
+                    for (int u = 0; u < RUN; u++) {
+                        const bool in = p + u * rsg < end;
+                        const uint32_t o = bo + jj[u];
+                        atomicSub(&cells[in ? (o >> 1) : own], in ? ((o & 1) ? 0x10000u : 1u) : 0u);
+                    }
+                }
+                if (rsg >= 16) {
+                    int next = 0;
+                    if (sl == 0) next = (int)atomicAdd(&sh.next, 1u);
+                    e = __shfl(next, leader, 64);
+                } else {
+                    e += RNT / rsg;
                 }
             }
-            __syncthreads();  // ebeg / eend are rewritten by the next batch
+            __syncthreads();  // ebeg / eend / next are rewritten by the next batch
         }
     }
+}
 
+// The staged words back to the layout (cap applied first when set): whole
+// words inside [g0, g1) as 16-byte stores, the partial first and last words
+// cell by cell (their other cells belong to other items).
+__device__ __forceinline__ void rare_write_back(const RareJob& jb, const RareItem& it, RareShared& sh) {
+    const int tid = threadIdx.x;
     if (jb.cap > 0) {  // main.rs:251-253 on the final values
-        uint16_t* c16 = reinterpret_cast<uint16_t*>(cells4);
-        for (int k = tid; k < nq * 8; k += RNT) c16[k] = min(c16[k], (uint16_t)jb.cap);
+        uint16_t* c16 = reinterpret_cast<uint16_t*>(sh.cells4);
+        for (int k = tid; k < it.nq * 8; k += RNT) c16[k] = min(c16[k], (uint16_t)jb.cap);
         __syncthreads();
     }
-    uint4* __restrict__ dst = reinterpret_cast<uint4*>(out + a0);
-    for (int k = tid; k < nq; k += RNT) {
-        const size_t lo = a0 + 8 * (size_t)k;  // u16 index of the word's first cell
-        if (lo >= g0 && lo + 8 <= g1) {
-            dst[k] = cells4[k];
+    uint16_t* __restrict__ out = jb.out;
+    uint4* __restrict__ dst = reinterpret_cast<uint4*>(out + it.a0);
+    for (int k = tid; k < it.nq; k += RNT) {
+        const size_t lo = it.a0 + 8 * (size_t)k;  // u16 index of the word's first cell
+        if (lo >= it.g0 && lo + 8 <= it.g1) {
+            dst[k] = sh.cells4[k];
         } else {
-            const uint16_t* c16 = reinterpret_cast<const uint16_t*>(&cells4[k]);
+            const uint16_t* c16 = reinterpret_cast<const uint16_t*>(&sh.cells4[k]);
             for (int h = 0; h < 8; h++)
-                if (lo + h >= g0 && lo + h < g1) out[lo + h] = c16[h];
+                if (lo + h >= it.g0 && lo + h < it.g1) out[lo + h] = c16[h];
         }
+    }
+}
+
+// Items b0, b0 + step, ... < nitems of job jb, one after the other in this
+// workgroup.  The next item's staging loads are issued before the current
+// item is written back, so their latency is spent under the write-back
+// instead of in front of the next item's atomics.
+template <typename M>
+__device__ __forceinline__ void rare_items(const RareJob& jb, const M* __restrict__ gmem, long long b0,
+                                           long long nitems, long long step, RareShared& sh) {
+    RareItem cur, nxt;
+    long long b = b0;
+    while (b < nitems && !rare_item(jb, b, cur)) b += step;
+    if (b >= nitems) return;
+    constexpr auto SEQ = std::make_index_sequence<RQ>();
+    v4i v[RQ];
+    rare_stage_issue(jb, cur, v, SEQ);
+    while (true) {
+        rare_stage_land(cur, v, sh, SEQ);
+        rare_ranges(jb, gmem, cur, sh);
+        long long nb = b + step;
+        while (nb < nitems && !rare_item(jb, nb, nxt)) nb += step;
+        if (nb < nitems) rare_stage_issue(jb, nxt, v, SEQ);
+        rare_write_back(jb, cur, sh);
+        if (nb >= nitems) return;
+        __syncthreads();  // the write-back's LDS reads before the next item lands
+        b = nb;
+        cur = nxt;
     }
 }
 
 // Member lists are read as u16 when every row index fits: half the bytes of
 // the pass's dominant (L2-missing) reads.
-// Workgroup b of job jb -> its rows and chunk.
-__device__ __forceinline__ void rare_rows_body(const RareJob& jb, long long b, RareShared& sh) {
-    int i0, i1, ch = 0;
-    if (jb.G > 1) {
-        const int s = (int)(b / jb.gps), g = (int)(b % jb.gps);
-        const int sr0 = jb.r0 + s * jb.S;
-        i0 = sr0 + g * jb.G;
-        i1 = min(min(i0 + jb.G, sr0 + jb.S), jb.n);
-        if (i0 >= i1) return;
-    } else {
-        i0 = jb.r0 + (int)(b / jb.chunks);
-        i1 = i0 + 1;
-        ch = (int)(b % jb.chunks);
-    }
-    if (jb.m16) rare_rows_impl(jb, static_cast<const uint16_t*>(jb.gmem), i0, i1, ch, sh);
-    else rare_rows_impl(jb, static_cast<const uint32_t*>(jb.gmem), i0, i1, ch, sh);
+__device__ __forceinline__ void rare_items_any(const RareJob& jb, long long b0, long long nitems, long long step,
+                                               RareShared& sh) {
+    if (jb.m16) rare_items(jb, static_cast<const uint16_t*>(jb.gmem), b0, nitems, step, sh);
+    else rare_items(jb, static_cast<const uint32_t*>(jb.gmem), b0, nitems, step, sh);
 }
 
-// One job.
-__global__ void __launch_bounds__(RNT) sub_rare_rows(RareJob jb) {
+// One job, a persistent grid: workgroup w takes items w, w + grid, ...
+__global__ void __launch_bounds__(RNT) sub_rare_rows(RareJob jb, long long nitems) {
     __shared__ RareShared sh;
-    rare_rows_body(jb, blockIdx.x, sh);
+    rare_items_any(jb, blockIdx.x, nitems, gridDim.x, sh);
 }
 
-// Many jobs in one launch: workgroup b belongs to the last job whose
-// block0 <= b (jobs ordered by block0).
+// Many jobs in one launch: workgroup b takes item b - block0 of the last job
+// whose block0 <= b (jobs ordered by block0).
 __global__ void __launch_bounds__(RNT) sub_rare_rows_batch(const RareJob* __restrict__ jobs, int njobs,
                                                             uint16_t* base, const size_t* __restrict__ outoff) {
     __shared__ RareShared sh;
@@ -891,14 +974,28 @@ __global__ void __launch_bounds__(RNT) sub_rare_rows_batch(const RareJob* __rest
     }
     RareJob jb = jobs[lo];
     jb.out = base + outoff[jb.d];
-    rare_rows_body(jb, b - jb.block0, sh);
+    rare_items_any(jb, b - jb.block0, b - jb.block0 + 1, 1, sh);
 }
 
 size_t up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
-// Workgroup layout of a rare job over `rows` rows from jb.r0: row groups
-// when G >= 2 rows of the widest stripe fit one chunk, else one row per
-// workgroup in column chunks.  Returns the number of workgroups.
+// Workgroups of a persistent rare launch on the current device: one per CU
+// (the LDS of one workgroup leaves no room for a second), at most one per item.
+unsigned rare_grid(long long items) {
+    static std::atomic<int> cus[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    int c = cus[dev].load(std::memory_order_relaxed);
+    if (c <= 0) {
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+        cus[dev].store(c, std::memory_order_relaxed);
+    }
+    return (unsigned)std::max<long long>(1, std::min<long long>(items, c));
+}
+
+// Work items of a rare job over `rows` rows from jb.r0: row groups when
+// G >= 2 rows of the widest stripe fit one chunk, else one row per item in
+// column chunks.  Returns the number of items.
 long long rare_layout(RareJob& jb, int rows) {
     const int pitch = jb.rect ? jb.nB : jb.n - jb.r0;  // the widest row of the layout
     jb.G = jb.rect ? 1 : std::max(1, RCW / std::max(pitch, 1));
@@ -1492,8 +1589,8 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         RareJob jb = {d_out, n, 0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16, ds->cap, 1, 0,
                       lanes_per_range(ds->n_rare, ds->n_rentries)};
-        const long long blocks = rare_layout(jb, n);
-        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)blocks), dim3(RNT), 0, st, jb);
+        const long long items = rare_layout(jb, n);
+        hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(items)), dim3(RNT), 0, st, jb, items);
         HK(hipGetLastError());
     }
     return 0;
@@ -1955,8 +2052,8 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);
     if (ne) {
         RareJob jb = {sl.ddist, q, 0, q, 1, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n, db->m16, 0, 1, 0, 32};
-        const long long blocks = rare_layout(jb, q);
-        hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)blocks), dim3(RNT), 0, sl.st, jb);
+        const long long items = rare_layout(jb, q);
+        hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(items)), dim3(RNT), 0, sl.st, jb, items);
     }
     const int tchunks = (db->n + TKC - 1) / TKC;
     hipLaunchKernelGGL(top_k_part, dim3((unsigned)((size_t)q * tchunks)), dim3(256), 0, sl.st, sl.ddist, db->n,
@@ -2241,8 +2338,8 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
             if (ds->n_rare && n - r0 > 1) {
                 RareJob jb = {dU[b], n, r0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16, ds->cap, 1,
                               0, lanes_per_range(ds->n_rare, ds->n_rentries)};
-                const long long blocks = rare_layout(jb, rows);
-                hipLaunchKernelGGL(sub_rare_rows, dim3((unsigned)blocks), dim3(RNT), 0, st[b], jb);
+                const long long items = rare_layout(jb, rows);
+                hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(items)), dim3(RNT), 0, st[b], jb, items);
                 HK(hipGetLastError());
             }
             HK(hipMemcpyAsync(hU[b], dU[b], (size_t)rows * w * 2, hipMemcpyDeviceToHost, st[b]));
