@@ -219,7 +219,7 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
         return;
     }
 
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm0 = (wave / WAVES_N) * WM, wn0 = (wave % WAVES_N) * WN;
     const int l32 = lane & 31, g = lane >> 5;
 
@@ -414,8 +414,29 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
                 }
             }
     };
-    if (a.cap > 0) store_all(std::true_type{});
-    else store_all(std::false_type{});
+    // Interior tiles (every row < rows, every column < w and, in the
+    // triangle, every column right of every row) need no per-cell test: one
+    // convert and one store per cell, addressed as a wavefront-uniform base
+    // plus a 32-bit lane offset; the row offsets are wavefront-uniform too.
+    const bool interior = rt0 + BM <= rows && ct0 + BM <= w && (a.rect || ct0 >= rt0 + BM);
+    if (interior && a.cap <= 0) {
+        char* __restrict__ wb0 = reinterpret_cast<char*>(out + (size_t)(rt0 + wm0) * w + (ct0 + wn0));
+        const uint32_t lo = 2u * ((uint32_t)l32 + (uint32_t)(g * 4) * (uint32_t)w);  // byte offsets below 2^32
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const uint32_t ro = lo + 2u * (uint32_t)(i * 32 + (e >> 2) * 8 + (e & 3)) * (uint32_t)w;
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    *reinterpret_cast<uint16_t*>(wb0 + (ro + j * 64)) =
+                        (uint16_t)(FP4 ? (uint32_t)(int)acc[i][j][e] : (uint32_t)(-(int)acc[i][j][e]));
+            }
+    } else if (a.cap > 0) {
+        store_all(std::true_type{});
+    } else {
+        store_all(std::false_type{});
+    }
 }
 
 // Tile shapes used: 256 x 256 by 8 wavefronts of 128 x 64 (two per SIMD, so
