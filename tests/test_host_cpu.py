@@ -7,6 +7,7 @@ launched here."""
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -184,3 +185,13 @@ def test_query_codec_rejects_short_buffer(cgh, workloads):
     assert lib.cgh_codec_encode(codec.handle, cgh._ptr(q), 5, cgh._ptr(O), cgh._ptr(P), cgh._ptr(rs), cgh._ptr(re),
                                 re.size, err, 512) == -1
     assert b"rare entries" in err.value
+
+
+def test_bench_one_gpu_only():
+    """bench.py measures one GPU: any other --gpus exits 2 with a message,
+    before touching a device."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2", "--steps", "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
