@@ -879,11 +879,9 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
             }
             if (tid == 0) sh.next = RNT / jb.rsg;  // ranges below this are taken without asking
             __syncthreads();
-            // Subgroup sub starts with range sub.  With long ranges (16 lanes
-            // or more each) it then takes the next free one from sh.next, so a
-            // few long ranges do not leave the other subgroups idle at the
-            // barrier; short ones go round-robin (a take per range would cost
-            // more than the imbalance).  Loads and atomics are issued for every
+            // Subgroup sub starts with range sub and then takes the next free
+            // one from sh.next, so a few long ranges do not leave the other
+            // subgroups idle at the barrier.  Loads and atomics are issued for every
             // slot: past the range end a lane reloads the last member and
             // adds 0 to a word of its own (no branches, no waits on them).
             const int rsg = jb.rsg, sub = tid / rsg, sl = tid % rsg, leader = (tid & 63) - sl;
@@ -903,13 +901,9 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
                         atomicSub(&cells[in ? (o >> 1) : own], in ? ((o & 1) ? 0x10000u : 1u) : 0u);
                     }
                 }
-                if (rsg >= 16) {
-                    int next = 0;
-                    if (sl == 0) next = (int)atomicAdd(&sh.next, 1u);
-                    e = __shfl(next, leader, 64);
-                } else {
-                    e += RNT / rsg;
-                }
+                int next = 0;
+                if (sl == 0) next = (int)atomicAdd(&sh.next, 1u);
+                e = __shfl(next, leader, 64);
             }
             __syncthreads();  // ebeg / eend / next are rewritten by the next batch
         }
