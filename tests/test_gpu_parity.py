@@ -230,29 +230,28 @@ def test_batch_matches_oracle_and_graph_replays(cgh, gpu, workloads, kernel):
 
 
 def test_rare_pass_many_chunks(cgh, gpu):
-    """Rows longer than one LDS chunk (n > 32,768): chunk clipping of the
-    partner ranges, checked on a row sample against the oracle."""
+    """n = 70,000 > 65,536: member lists stored as u32, and the rows with more
+    than 65,536 cells right of the diagonal (i < 4,463) split into two LDS
+    chunks, each clipping every partner range by binary search; rows on both
+    sides of that split and of the stripe edges checked against the oracle."""
     rng = np.random.default_rng(12)
-    n, L = 40000, 24
-    X = rng.integers(1, 300, size=(n, L)).astype(np.uint16)   # ~133 copies per allele: rare at tau=4096
+    n, L = 70000, 24
+    X = rng.integers(1, 300, size=(n, L)).astype(np.uint16)   # ~230 copies per allele: rare at tau=4096
     X[rng.random((n, L)) < 0.05] = 0
     S = 4096
     enc = cgh.Encoded(X, 4096, 256)
-    assert enc.enc.n_rare > 5 * 10**7
+    assert enc.enc.n_rare > 10**8
     ds = cgh.Dataset(enc)
     out = gpu.zeros(cgh.upper_elems(n, S), dtype=gpu.int16, device="cuda")
-    print("rare many chunks", flush=True)
+    print("rare many chunks, u32 members", flush=True)
     ds.upper(S, out.data_ptr(), gpu.cuda.current_stream().cuda_stream)
     gpu.cuda.synchronize()
-    flat = out.cpu().numpy().view(np.uint16)
     ds.close()
-    rows = [0, 1, 2, 4095, 4096, 8191, 20000, 39998]
-    ref = oracle.brute_rows(X, 0, 1)  # warm the library
-    for i in rows:
+    for i in [0, 1, 2, 4095, 4096, 4461, 4462, 4463, 4464, 8191, 40000, 69998]:
         s = i // S
         r0 = s * S
         w = n - r0
         off = cgh.stripe_offset(n, S, s) + (i - r0) * w
-        got = flat[off + (i + 1 - r0): off + w]
+        got = out[off + (i + 1 - r0): off + w].cpu().numpy().view(np.uint16)
         ref = oracle.brute_rows(X, i, i + 1)[0, i + 1:]
         assert np.array_equal(got, ref), i
