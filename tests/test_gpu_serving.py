@@ -93,3 +93,27 @@ def test_server_database_smaller_than_top_k(cgh, gpu, workloads):
     assert np.all(top_d[:, 7:] == 0xFFFF) and np.all(top_i[:, 7:] == 0xFFFFFFFF)
     sv.close()
     db.close()
+
+
+@pytest.mark.parametrize("tau", [64, 10**6])
+def test_server_database_beyond_u16_rows(cgh, gpu, workloads, tau):
+    """A 70,000-row database: member lists as u32 and every query row split
+    into two LDS chunks of the rare pass (partner ranges clipped per chunk);
+    distances and top-10 against the oracle."""
+    n_db, L, sizes = 70000, 40, [3, 20]
+    db_calls, q_calls = workloads.serving_sets(n_db, L, sum(sizes), seed=31)
+    X_all = oracle.dense_ranks(np.vstack([db_calls, q_calls]))
+    ref = oracle.brute_rows(X_all, n_db, X_all.shape[0])[:, :n_db]
+    db = cgh.Database(db_calls, tau)
+    sv = cgh.Server(db, slots=2, max_batch=64)
+    print(f"server 70000-row database tau={tau}", flush=True)
+    bounds = np.cumsum([0] + sizes)
+    tickets = [sv.submit(q_calls[bounds[k]:bounds[k + 1]]) for k in range(len(sizes))]
+    for k, t in enumerate(tickets):
+        dist, top_d, top_i, _ = sv.wait(t)
+        want = ref[bounds[k]:bounds[k + 1]]
+        td, ti = expected_top(want)
+        assert np.array_equal(dist, want), k
+        assert np.array_equal(top_d, td) and np.array_equal(top_i, ti), k
+    sv.close()
+    db.close()
