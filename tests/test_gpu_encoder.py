@@ -82,3 +82,29 @@ def test_device_encoder_long_tail_full_size(cgh, gpu, workloads):
     a = upper(cgh, gpu, host, n, S)
     b = upper(cgh, gpu, dev, n, S)
     assert bool(gpu.equal(a, b))
+
+
+def test_device_encoder_beyond_u16_rows(cgh, gpu):
+    """n = 70,000 > 65,536: the device encoder writes u32 member lists (and
+    the rare pass splits the widest rows into two chunks); ranks equal the
+    host's and sampled rows equal the oracle's."""
+    rng = np.random.default_rng(41)
+    n, L, S = 70000, 24, 4096
+    calls = rng.integers(1, 300, size=(n, L)).astype(np.uint64) * 1000003
+    calls[rng.random((n, L)) < 0.05] = 0
+    out_bytes = cgh.upper_elems(n, S) * 2
+    print("device encoder 70000 rows, tau 4096", flush=True)
+    ds, tau_used, Xd = cgh.Dataset.encode(calls, 4096, out_bytes, want_ranks=True)
+    X = cgh.dense_ranks(calls)
+    assert np.array_equal(Xd, X) and tau_used == 4096
+    assert ds.info()["n_rare"] > 10**8
+    out = gpu.zeros(cgh.upper_elems(n, S), dtype=gpu.int16, device="cuda")
+    ds.upper(S, out.data_ptr(), gpu.cuda.current_stream().cuda_stream, 0)
+    gpu.cuda.synchronize()
+    for i in [0, 4462, 4463, 35000, 69998]:
+        s = i // S
+        r0 = s * S
+        off = cgh.stripe_offset(n, S, s) + (i - r0) * (n - r0)
+        got = out[off + (i + 1 - r0): off + (n - r0)].cpu().numpy().view(np.uint16)
+        assert np.array_equal(got, oracle.brute_rows(X, i, i + 1)[0, i + 1:]), i
+    ds.close()
