@@ -881,29 +881,63 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
             __syncthreads();
             // Subgroup sub starts with range sub and then takes the next free
             // one from sh.next, so a few long ranges do not leave the other
-            // subgroups idle at the barrier.  Loads and atomics are issued for every
-            // slot: past the range end a lane reloads the last member and
-            // adds 0 to a word of its own (no branches, no waits on them).
+            // subgroups idle at the barrier.  A range is walked in units of
+            // rsg * RUN members, and the members of the next unit are loaded
+            // while the current unit's atomics are issued, so the member-load
+            // latency overlaps the LDS work instead of adding to it.  Loads and
+            // atomics are issued for every slot: past the unit end a lane
+            // reloads the last member and adds 0 to a word of its own (no
+            // branches, so the load waits are fixed counts).
             const int rsg = jb.rsg, sub = tid / rsg, sl = tid % rsg, leader = (tid & 63) - sl;
             const uint32_t own = (uint32_t)(reinterpret_cast<uint32_t*>(sh.spare) - cells) + (tid & 63);
-            const uint32_t bo = (uint32_t)(base - it.a0);  // LDS u16 offset of cell (i, 0)
-            for (int e = sub; e < nb;) {
-                const uint32_t b = ebeg[e], end = eend[e];
-                for (uint32_t p0 = b; p0 < end; p0 += rsg * RUN) {
-                    const uint32_t p = p0 + sl;
-                    uint32_t jj[RUN];
+            const uint32_t bo = (uint32_t)(base - it.a0);  // LDS u16 offset of cell (i, 0), mod 2^32
+            // current unit: members [cp0, min(cp0 + step, cend)) of its range
+            const uint32_t step = (uint32_t)(rsg * RUN);
+            uint32_t cp0 = 0, cend = 0;
+            if (sub < nb) {
+                cp0 = ebeg[sub];
+                cend = eend[sub];
+            }
+            uint32_t ja[RUN];
 #pragma unroll
-                    for (int u = 0; u < RUN; u++) jj[u] = (uint32_t)gmem[min(p + u * rsg, end - 1)];
+            for (int u = 0; u < RUN; u++)
+                ja[u] = (uint32_t)gmem[min(cp0 + sl + u * rsg, max(cend, 1u) - 1)];
+            for (;;) {
+                uint32_t np0, nend;
+                bool more = true;
+                if (cp0 + step < cend) {
+                    np0 = cp0 + step;
+                    nend = cend;
+                } else {
+                    int nx = 0;
+                    if (sl == 0) nx = (int)atomicAdd(&sh.next, 1u);
+                    nx = __shfl(nx, leader, 64);
+                    np0 = 0;
+                    nend = 0;
+                    if (nx < nb) {
+                        np0 = ebeg[nx];
+                        nend = eend[nx];
+                    } else {
+                        more = false;
+                    }
+                }
+                uint32_t jn[RUN];
+#pragma unroll
+                for (int u = 0; u < RUN; u++)
+                    jn[u] = (uint32_t)gmem[min(np0 + sl + u * rsg, max(nend, 1u) - 1)];
+                if (cp0 < cend) {
 #pragma unroll
                     for (int u = 0; u < RUN; u++) {
-                        const bool in = p + u * rsg < end;
-                        const uint32_t o = bo + jj[u];
+                        const bool in = cp0 + sl + u * rsg < cend;
+                        const uint32_t o = bo + ja[u];
                         atomicSub(&cells[in ? (o >> 1) : own], in ? ((o & 1) ? 0x10000u : 1u) : 0u);
                     }
                 }
-                int next = 0;
-                if (sl == 0) next = (int)atomicAdd(&sh.next, 1u);
-                e = __shfl(next, leader, 64);
+                if (!more) break;
+                cp0 = np0;
+                cend = nend;
+#pragma unroll
+                for (int u = 0; u < RUN; u++) ja[u] = jn[u];
             }
             __syncthreads();  // ebeg / eend / next are rewritten by the next batch
         }
