@@ -195,3 +195,16 @@ def test_bench_one_gpu_only():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+@pytest.mark.parametrize("tau", [2, 8, 64, 10**6])
+def test_member_range_count(cgh, tau):
+    """The tau model counts one member range per member of a rare group but
+    the group's last (sum of c - 1 over the allele counts 2 <= c < tau that
+    cgh_choose_tau reads): that is the size of the encoder's row index."""
+    A, _ = oracle.check_make(700, 410, 2, True)
+    X = oracle.dense_ranks(A.astype(np.uint64))
+    enc = cgh.Encoded(X, tau, 256)
+    cnt = cgh.allele_counts(X).astype(np.int64)
+    c = cnt[(cnt >= 2) & (cnt < tau)]
+    assert int(enc.enc.n_rentries) == int((c - 1).sum())
