@@ -229,6 +229,41 @@ def test_batch_matches_oracle_and_graph_replays(cgh, gpu, workloads, kernel):
         s.close()
 
 
+def test_batch_with_a_dataset_beyond_u16_rows(cgh, gpu):
+    """A batch holding a 70,000-row dataset (u32 member lists, wide rows in
+    two LDS chunks) beside a small one: the batched launch equals the
+    per-dataset launches, and sampled rows equal the oracle."""
+    rng = np.random.default_rng(19)
+    big = rng.integers(1, 300, size=(70000, 24)).astype(np.uint16)
+    big[rng.random(big.shape) < 0.05] = 0
+    small = cgh.dense_ranks(np.random.default_rng(3).integers(1, 40, size=(900, 60)).astype(np.uint64))
+    S = 4096
+    sets = [cgh.Dataset(cgh.Encoded(big, 4096, 256)), cgh.Dataset(cgh.Encoded(small, 64, 256))]
+    b = cgh.Batch(sets, S)
+    out = gpu.zeros(b.out_elems(), dtype=gpu.int16, device="cuda")
+    print("batch with a 70000-row dataset", flush=True)
+    b.run(out.data_ptr(), gpu.cuda.current_stream().cuda_stream, 0, graph=False)
+    plain = gpu.zeros_like(out)
+    for d, ds in enumerate(sets):
+        n = (big if d == 0 else small).shape[0]
+        view = plain[b.offset(d): b.offset(d) + cgh.upper_elems(n, S)]
+        ds.upper(S, view.data_ptr(), gpu.cuda.current_stream().cuda_stream, 0)
+    gpu.cuda.synchronize()
+    for d, X in enumerate((big, small)):
+        n = X.shape[0]
+        seg = slice(b.offset(d), b.offset(d) + cgh.upper_elems(n, S))
+        assert bool(gpu.equal(out[seg], plain[seg])), d
+        for i in ([0, 4462, 4463, 35000, 69998] if d == 0 else [0, 450, 898]):
+            s = i // S
+            r0 = s * S
+            off = b.offset(d) + cgh.stripe_offset(n, S, s) + (i - r0) * (n - r0)
+            got = out[off + (i + 1 - r0): off + (n - r0)].cpu().numpy().view(np.uint16)
+            assert np.array_equal(got, oracle.brute_rows(X, i, i + 1)[0, i + 1:]), (d, i)
+    b.close()
+    for ds in sets:
+        ds.close()
+
+
 def test_rare_pass_many_chunks(cgh, gpu):
     """n = 70,000 > 65,536: member lists stored as u32, and the rows with more
     than 65,536 cells right of the diagonal (i < 4,463) split into two LDS
