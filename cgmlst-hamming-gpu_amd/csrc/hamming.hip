@@ -744,8 +744,6 @@ struct RareJob {
     const uint32_t* rstart;    // n + 1 offsets into rentry
     const uint64_t* rentry;    // partner ranges of gmem
     const void* gmem;          // group members: u16 if m16 (every row < 65536), else u32
-    long long block0;          // first workgroup of this job in a batched launch
-    int d;                     // dataset of this job in a batched launch
     int rect, nB;              // rectangular form: rows against nB columns
     int m16;
     int cap;                   // > 0: staged cells are written back as min(value, cap)
@@ -1010,21 +1008,6 @@ __global__ void __launch_bounds__(RNT) sub_rare_rows(RareJob jb, long long nitem
     rare_items_any(jb, blockIdx.x, nitems, gridDim.x, sh);
 }
 
-// Many jobs in one launch: workgroup b takes item b - block0 of the last job
-// whose block0 <= b (jobs ordered by block0).
-__global__ void __launch_bounds__(RNT) sub_rare_rows_batch(const RareJob* __restrict__ jobs, int njobs,
-                                                            uint16_t* base, const size_t* __restrict__ outoff) {
-    __shared__ RareShared sh;
-    const long long b = blockIdx.x;
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].block0 <= b) lo = mid; else hi = mid - 1;
-    }
-    RareJob jb = jobs[lo];
-    jb.out = base + outoff[jb.d];
-    rare_items_any(jb, b - jb.block0, b - jb.block0 + 1, 1, sh);
-}
 
 size_t up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
@@ -1636,7 +1619,7 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
         HK(launch_tiles(ds_kernel(ds, kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
                          S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
-        RareJob jb = {d_out, n, 0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16, ds->cap, 1, 0,
+        RareJob jb = {d_out, n, 0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, ds->m16, ds->cap, 1, 0,
                       lanes_per_range(ds->n_rare, ds->n_rentries)};
         const long long items = rare_layout(jb, n);
         hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(items)), dim3(RNT), 0, st, jb, items);
@@ -1666,9 +1649,9 @@ struct cgh_batch {
     size_t* doffs = NULL;
     int4* dtiles[2] = {NULL, NULL};     // work lists of the 256 and 128 tile kernels
     int ntiles[2] = {0, 0};
-    RareJob* drare = NULL;
-    int nrare = 0;
-    long long rare_blocks = 0;
+    std::vector<RareJob> rare;          // one rare-pass launch per dataset with rare pairs
+    std::vector<int> rare_ds;           // its dataset
+    std::vector<long long> rare_items;  // its work items
     int* dcount = NULL;                 // two work-queue counters
     hipGraphExec_t exec = NULL;         // captured launch sequence
     uint16_t* graph_out = NULL;
@@ -1684,7 +1667,6 @@ extern "C" void cgh_batch_free(cgh_batch* b) {
     if (b->doffs) (void)hipFree(b->doffs);
     for (int k = 0; k < 2; k++)
         if (b->dtiles[k]) (void)hipFree(b->dtiles[k]);
-    if (b->drare) (void)hipFree(b->drare);
     if (b->dcount) (void)hipFree(b->dcount);
     delete b;
 }
@@ -1693,7 +1675,6 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
                                 char* err, int errlen) {
     cgh_batch* b = new cgh_batch();
     std::vector<TileJob> jobs((size_t)count);
-    std::vector<RareJob> rjobs;
     std::vector<std::pair<long long, int4>> lists[2];  // (cost, tile) per kernel
     *out = NULL;
     b->device = device;
@@ -1730,13 +1711,13 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
                 for (int J = 0; J * tl < w; J++) lists[k].push_back({J >= I ? cost : 0, make_int4(d, s, I, J)});
         }
         if (ds->n_rare && ds->n > 1) {
-            RareJob r = {NULL, ds->n, 0, S, 1, ds->drstart, ds->drentry, ds->dgmem, b->rare_blocks, d, 0, 0, ds->m16,
-                         ds->cap, 1, 0, lanes_per_range(ds->n_rare, ds->n_rentries)};
-            b->rare_blocks += rare_layout(r, ds->n);
-            rjobs.push_back(r);
+            RareJob r = {NULL, ds->n, 0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, ds->m16, ds->cap, 1, 0,
+                         lanes_per_range(ds->n_rare, ds->n_rentries)};
+            b->rare_items.push_back(rare_layout(r, ds->n));
+            b->rare.push_back(r);
+            b->rare_ds.push_back(d);
         }
     }
-    b->nrare = (int)rjobs.size();
     HK(hipSetDevice(device));
     HK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
     HK(hipMalloc(&b->djobs, sizeof(TileJob) * (size_t)count));
@@ -1755,10 +1736,6 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         if (t.empty()) continue;
         HK(hipMalloc(&b->dtiles[k], sizeof(int4) * t.size()));
         HK(hipMemcpy(b->dtiles[k], t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
-    }
-    if (b->nrare) {
-        HK(hipMalloc(&b->drare, sizeof(RareJob) * rjobs.size()));
-        HK(hipMemcpy(b->drare, rjobs.data(), sizeof(RareJob) * rjobs.size(), hipMemcpyHostToDevice));
     }
     HK(hipMalloc(&b->dcount, 2 * sizeof(int)));
     *out = b;
@@ -1803,9 +1780,12 @@ hipError_t batch_enqueue(cgh_batch* b, uint16_t* d_out, hipStream_t st, int kern
             hipLaunchKernelGGL((mfma_tile_queue<128, 64, 64, false>), dim3(cus * 2), dim3(256), 0, st, b->djobs,
                                d_out, b->doffs, b->dtiles[1], b->ntiles[1], b->dcount + 1);
     }
-    if (b->nrare)
-        hipLaunchKernelGGL(sub_rare_rows_batch, dim3((unsigned)b->rare_blocks), dim3(RNT), 0, st, b->drare, b->nrare,
-                           d_out, b->doffs);
+    // rare pairs: one persistent launch per dataset, after every product
+    for (size_t k = 0; k < b->rare.size(); k++) {
+        RareJob jb = b->rare[k];
+        jb.out = d_out + b->offs[b->rare_ds[k]];
+        hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(b->rare_items[k])), dim3(RNT), 0, st, jb, b->rare_items[k]);
+    }
     return hipGetLastError();
 }
 
@@ -2100,7 +2080,7 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     else if (fp4) hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, sl.st, a);
     else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);
     if (ne) {
-        RareJob jb = {sl.ddist, q, 0, q, 1, sl.drs, sl.dre, db->dqmem, 0, 0, 1, db->n, db->m16, 0, 1, 0, 32};
+        RareJob jb = {sl.ddist, q, 0, q, 1, sl.drs, sl.dre, db->dqmem, 1, db->n, db->m16, 0, 1, 0, 32};
         const long long items = rare_layout(jb, q);
         hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(items)), dim3(RNT), 0, sl.st, jb, items);
     }
@@ -2385,7 +2365,7 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
             HK(launch_tiles(ds_kernel(ds, CGH_KERNEL_AUTO), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2,
                             (int)ds->n_pad, n, r0, S, 1, dU[b], st[b], ds->drstart, ds->cap));
             if (ds->n_rare && n - r0 > 1) {
-                RareJob jb = {dU[b], n, r0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, 0, 0, ds->m16, ds->cap, 1,
+                RareJob jb = {dU[b], n, r0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, ds->m16, ds->cap, 1,
                               0, lanes_per_range(ds->n_rare, ds->n_rentries)};
                 const long long items = rare_layout(jb, rows);
                 hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(items)), dim3(RNT), 0, st[b], jb, items);
