@@ -1657,6 +1657,12 @@ struct cgh_batch {
     uint16_t* graph_out = NULL;
     hipStream_t graph_stream = NULL;
     int graph_kernel = -1;
+    // rare passes of different datasets run side by side on RSTREAMS streams
+    // (the caller's plus side ones), balanced by work items
+    static constexpr int RSTREAMS = 4;
+    hipStream_t side[RSTREAMS - 1] = {};
+    hipEvent_t fork = NULL, join[RSTREAMS - 1] = {};
+    std::vector<int> rare_lane;  // stream of each rare job (0 = the caller's)
 };
 
 extern "C" void cgh_batch_free(cgh_batch* b) {
@@ -1668,6 +1674,11 @@ extern "C" void cgh_batch_free(cgh_batch* b) {
     for (int k = 0; k < 2; k++)
         if (b->dtiles[k]) (void)hipFree(b->dtiles[k]);
     if (b->dcount) (void)hipFree(b->dcount);
+    for (int k = 0; k < cgh_batch::RSTREAMS - 1; k++) {
+        if (b->side[k]) (void)hipStreamDestroy(b->side[k]);
+        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     delete b;
 }
 
@@ -1738,6 +1749,25 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         HK(hipMemcpy(b->dtiles[k], t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
     }
     HK(hipMalloc(&b->dcount, 2 * sizeof(int)));
+    if (!b->rare.empty()) {  // largest first, each to the least loaded stream
+        std::vector<size_t> order(b->rare.size());
+        for (size_t k = 0; k < order.size(); k++) order[k] = k;
+        std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return b->rare_items[x] > b->rare_items[y]; });
+        long long load[cgh_batch::RSTREAMS] = {};
+        b->rare_lane.assign(b->rare.size(), 0);
+        for (size_t k : order) {
+            int best = 0;
+            for (int l = 1; l < cgh_batch::RSTREAMS; l++)
+                if (load[l] < load[best]) best = l;
+            b->rare_lane[k] = best;
+            load[best] += b->rare_items[k];
+        }
+        for (int k = 0; k < cgh_batch::RSTREAMS - 1; k++) {
+            HK(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking));
+            HK(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming));
+        }
+        HK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    }
     *out = b;
     return 0;
 fail:
@@ -1780,12 +1810,24 @@ hipError_t batch_enqueue(cgh_batch* b, uint16_t* d_out, hipStream_t st, int kern
             hipLaunchKernelGGL((mfma_tile_queue<128, 64, 64, false>), dim3(cus * 2), dim3(256), 0, st, b->djobs,
                                d_out, b->doffs, b->dtiles[1], b->ntiles[1], b->dcount + 1);
     }
-    // rare pairs: one persistent launch per dataset, after every product
+    // rare pairs: one persistent launch per dataset, after every product,
+    // spread over the caller's stream and the side streams (fork / join by
+    // events, which a stream capture follows into the graph)
+    if (b->rare.empty()) return hipGetLastError();
+    hipError_t e = hipEventRecord(b->fork, st);
+    for (int k = 0; k < cgh_batch::RSTREAMS - 1 && e == hipSuccess; k++) e = hipStreamWaitEvent(b->side[k], b->fork, 0);
+    if (e != hipSuccess) return e;
     for (size_t k = 0; k < b->rare.size(); k++) {
         RareJob jb = b->rare[k];
         jb.out = d_out + b->offs[b->rare_ds[k]];
-        hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(b->rare_items[k])), dim3(RNT), 0, st, jb, b->rare_items[k]);
+        const hipStream_t sk = b->rare_lane[k] ? b->side[b->rare_lane[k] - 1] : st;
+        hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(b->rare_items[k])), dim3(RNT), 0, sk, jb, b->rare_items[k]);
     }
+    for (int k = 0; k < cgh_batch::RSTREAMS - 1 && e == hipSuccess; k++) {
+        e = hipEventRecord(b->join[k], b->side[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, b->join[k], 0);
+    }
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
