@@ -244,10 +244,13 @@ size_t cgh_batch_out_offset(const cgh_batch* b, int d);
 int cgh_batch_tile(const cgh_batch* b, int d);
 
 /* Enqueue the whole batch on `stream`: one persistent product launch per
- * tile size pulling tiles from the queue, then one rare-pair launch over all
- * datasets.  use_graph != 0 (with a non-NULL stream) captures that sequence
- * into a HIP graph on the first call and replays it while d_out, stream and
- * kernel stay the same.  d_out: cgh_batch_out_elems(b) u16, 16-byte aligned. */
+ * tile size pulling tiles from the queue, then one rare-pair launch per
+ * dataset, spread over `stream` and three side streams of the batch (they
+ * fork from and join back into `stream` by events).  use_graph != 0 (with a
+ * non-NULL stream) captures that sequence into a HIP graph on the first call
+ * and replays it while d_out, stream and kernel stay the same.  d_out:
+ * cgh_batch_out_elems(b) u16, 16-byte aligned.  Calls on one batch object
+ * must not overlap (they share its side streams and events). */
 int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int kernel, int use_graph, char* err, int errlen);
 void cgh_batch_free(cgh_batch* b);
 
