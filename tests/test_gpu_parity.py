@@ -229,6 +229,40 @@ def test_batch_matches_oracle_and_graph_replays(cgh, gpu, workloads, kernel):
         s.close()
 
 
+def test_batch_with_caps(cgh, gpu, workloads):
+    """A batch of datasets with different -X caps (none, 5, 40): every
+    dataset's cells are min(H, cap), eager and replayed from the graph."""
+    shapes = [(900, 300, 11, 0), (1300, 500, 12, 5), (600, 800, 13, 40)]
+    S = 512
+    sets, Xs, caps = [], [], []
+    for n, L, seed, cap in shapes:
+        X = cgh.dense_ranks(workloads.zipf_iid(n, L, seed))
+        ds = cgh.Dataset(cgh.Encoded(X, 64, 256))
+        if cap:
+            ds.set_cap(cap)
+        sets.append(ds)
+        Xs.append(X)
+        caps.append(cap)
+    b = cgh.Batch(sets, S)
+    out = gpu.zeros(b.out_elems(), dtype=gpu.int16, device="cuda")
+    side = gpu.cuda.Stream()
+    for graph in (False, True):
+        out.fill_(0x2222)
+        gpu.cuda.synchronize()
+        print(f"batch with caps graph={graph}", flush=True)
+        b.run(out.data_ptr(), side.cuda_stream, 0, graph=graph)
+        gpu.cuda.synchronize()
+        flat = out.cpu().numpy().view(np.uint16)
+        for d, (X, cap) in enumerate(zip(Xs, caps)):
+            n = X.shape[0]
+            H = cgh.upper_to_square(flat[b.offset(d): b.offset(d) + cgh.upper_elems(n, S)], n, S)
+            ref = oracle.brute_full(X)
+            assert np.array_equal(H, np.minimum(ref, cap) if cap else ref), (d, graph)
+    b.close()
+    for ds in sets:
+        ds.close()
+
+
 def test_batch_with_a_dataset_beyond_u16_rows(cgh, gpu):
     """A batch holding a 70,000-row dataset (u32 member lists, wide rows in
     two LDS chunks) beside a small one: the batched launch equals the
