@@ -514,22 +514,34 @@ def upper_to_square(flat: np.ndarray, n: int, S: int) -> np.ndarray:
     return H
 
 
-def tch_run(enc: Encoded, S: int = 512, device: int = 0):
+def tch_run(enc: Encoded, S: int = 512, device: int = 0, keep_rows=None, shuffle_seed=None):
     """Run the reference-compatible tch_run on the encoding (K aligned to
-    1024) and return (H upper-triangle rows as a square matrix, gpu_secs,
-    rare_on_gpu)."""
+    1024).  Returns (H, gpu_secs, rare_on_gpu, stripes seen): H is the square
+    symmetric matrix, or with keep_rows a dict row -> its cells j > row.
+    shuffle_seed permutes each row's partner list first (the reference's CSR
+    order within a row is arbitrary)."""
     e = enc.enc
     if (e.kw1 * 32) % 1024 or (e.kw2 * 32) % 1024:
         raise CghError("tch_run needs an encoding with k_align=1024")
     n = e.n
     start, js = enc.rare_csr()
+    if shuffle_seed is not None:
+        rng = np.random.default_rng(shuffle_seed)
+        for i in range(n):
+            rng.shuffle(js[start[i]:start[i + 1]])
     js_buf = js if js.size else np.zeros(1, dtype=np.uint32)
-    H = np.zeros((n, n), dtype=np.uint16)
+    H = np.zeros((n, n), dtype=np.uint16) if keep_rows is None else {}
+    want = None if keep_rows is None else set(int(i) for i in keep_rows)
     seen = []
 
     def cb(ctx, r0, rows, w, stripe):
         blk = np.ctypeslib.as_array(stripe, shape=(rows * w,)).reshape(rows, w)
-        H[r0:r0 + rows, r0:] = blk
+        if want is None:
+            H[r0:r0 + rows, r0:] = blk
+        else:
+            for i in want:
+                if r0 <= i < r0 + rows:
+                    H[i] = blk[i - r0, i - r0 + 1:].copy()
         seen.append((r0, rows, w))
 
     ccb = TCH_CB(cb)
@@ -540,7 +552,8 @@ def tch_run(enc: Encoded, S: int = 512, device: int = 0):
                         _ptr(start), _ptr(js_buf), int(e.n_rare), S, ccb, None, C.byref(secs), C.byref(rare_gpu),
                         err, 512)
     _check(rc, err)
-    iu = np.triu_indices(n, 1)
-    H.T[iu] = H[iu]
-    np.fill_diagonal(H, 0)
+    if want is None:
+        iu = np.triu_indices(n, 1)
+        H.T[iu] = H[iu]
+        np.fill_diagonal(H, 0)
     return H, secs.value, bool(rare_gpu.value), seen

@@ -1,11 +1,14 @@
 // cli.cpp -- cgmlst-hamming-gpu: drop-in command line for the MI355X library.
 //
-// Same flags, defaults, output bytes and exit codes as the reference host
-// (tensorcore/rs/src/main.rs:19-82 flags, 267-491 main): allele-call TSV in,
-// full / lower-tri / upper-tri u16 distance TSV out (gzip with -z or *.gz).
-// Parsing, filters and dense ranks run on the host; the distances run on the
-// GPU through the library's tch_run, exactly the boundary the reference host
-// binds (main.rs:84-93).
+// Same flags, defaults, output bytes, exit codes and error texts as the
+// reference host (tensorcore/rs/src/main.rs:19-82 flags, 267-491 main):
+// allele-call TSV in, full / lower-tri / upper-tri u16 distance TSV out (gzip
+// with -z or *.gz).  Parsing and the completeness filters run on the host;
+// the encoder runs on the device (cgh_dataset_encode; the host encoder and
+// cgh_dataset_build when n x L reaches 2^31 calls), and the distances stream
+// back stripe by stripe through cgh_dataset_stream, which keeps the callback
+// contract of the reference's tch_run (main.rs:84-93) while reading the rare
+// pairs from the dataset's groups instead of a per-pair CSR.
 
 #include <errno.h>
 #include <omp.h>
@@ -330,6 +333,18 @@ bool valid_utf8(const std::string& s) {
     return true;
 }
 
+// MemAvailable of /proc/meminfo in bytes, 0 when unknown.
+double host_available_bytes() {
+    FILE* f = fopen("/proc/meminfo", "r");
+    if (!f) return 0;
+    char line[256];
+    double kb = 0;
+    while (fgets(line, sizeof line, f))
+        if (sscanf(line, "MemAvailable: %lf kB", &kb) == 1) break;
+    fclose(f);
+    return kb * 1024.0;
+}
+
 std::string io_error(int e) { return std::string(strerror(e)) + " (os error " + std::to_string(e) + ")"; }
 
 bool read_all(const std::string& path, std::string* out, int* eno) {
@@ -588,7 +603,7 @@ int main(int argc, char** argv) {
     if (args.input_sep.size() != 1) die("input separator must be a single character");
     const char sep = args.input_sep[0];
     if ((size_t)std::max(tch_device_count(), 0) <= args.device)
-        die("GPU device " + std::to_string(args.device) + " not available");
+        die("CUDA device " + std::to_string(args.device) + " not available");  // main.rs:283, kept verbatim
 
     // ---- load (main.rs:286-311) ----
     double t = now();
@@ -709,19 +724,46 @@ int main(int argc, char** argv) {
     t = now();
     char err[512] = {0};
     Matrix mx;
-    mx.init(n);  // 7.9 GB at 63k: every cell is written before its row is formatted
+    {  // the full n x n mirror (7.9 GB at 63k) must fit in host memory
+        const double need = (double)n * (double)n * 2.0, avail = host_available_bytes();
+        if (avail > 0 && need > avail) {
+            snprintf(msg, sizeof msg, "the %zu x %zu distance matrix needs %.2f GB of host memory, %.2f GB available",
+                     n, n, need / 1e9, avail / 1e9);
+            die(msg);
+        }
+    }
+    mx.init(n);  // every cell is written before its row is formatted
     wr.mx = &mx;
     wr.cells.build(args.output_sep, (uint32_t)std::max<size_t>(l, 1));
     cgh_dataset* ds = NULL;
     const size_t stripe = std::max<size_t>(args.stripe, 8) / 8 * 8;
+    // a device failure is reported the way main.rs:460 reports tch_run's
+    auto device_error = [&]() { return strncmp(err, "HIP error", 9) ? std::string(err) : std::string("CUDA: ") + err; };
     if (n >= 2 && l > 0) {
-        const size_t n_pad = (n + 255) / 256 * 256;
-        const size_t out_bytes = 2 * ((stripe * n_pad + 7) / 8 * 8) * 2;  // two stripe buffers
+        const size_t out_bytes = 3 * ((stripe * n + 7) / 8 * 8) * 2;  // the stripe ring's device slots
         const int tau_i = (int)std::min<size_t>(args.tau, (size_t)n + 1);  // counts never exceed n; 0 = auto
         int tau_used = 0;
-        if (cgh_dataset_encode((int)args.device, calls.data(), (int)n, (int)l, tau_i, out_bytes, log_line,
-                               &args.silent, &ds, &tau_used, NULL, err, sizeof err))
-            die(strncmp(err, "hip", 3) ? std::string(err) : std::string("GPU: ") + err);
+        if ((uint64_t)n * l < (1ull << 31)) {
+            if (cgh_dataset_encode((int)args.device, calls.data(), (int)n, (int)l, tau_i, out_bytes, log_line,
+                                   &args.silent, &ds, &tau_used, NULL, err, sizeof err))
+                die(device_error());
+        } else {  // beyond one device sort: ranks and rare groups on the host, O/P built on the device
+            std::vector<uint16_t> X(n * l);
+            if (cgh_dense_ranks(calls.data(), (int)n, (int)l, X.data(), err, sizeof err)) die(err);
+            tau_used = tau_i;
+            if (!tau_used) {
+                std::vector<uint32_t> cnt(cgh_allele_counts(X.data(), (int)n, (int)l, NULL, 0) + 1);
+                const size_t nc = cgh_allele_counts(X.data(), (int)n, (int)l, cnt.data(), cnt.size());
+                const size_t free_b = tch_free_vram((int)args.device);
+                tau_used = cgh_choose_tau(cnt.data(), nc, (int)n, (int)l, out_bytes, free_b, log_line, &args.silent);
+                if (!tau_used) {
+                    snprintf(msg, sizeof msg, "no tau fits in %.2f GB of free VRAM", free_b / 1e9);
+                    die(msg);
+                }
+            }
+            if (cgh_dataset_build((int)args.device, X.data(), (int)n, (int)l, tau_used, &ds, err, sizeof err))
+                die(device_error());
+        }
         size_t D = 0;
         uint64_t n_rare = 0;
         cgh_dataset_info(ds, NULL, NULL, NULL, &D, &n_rare);
@@ -742,7 +784,7 @@ int main(int argc, char** argv) {
         StripeCtx sc = {&mx, &wr};
         const int rc = cgh_dataset_stream(ds, (int)stripe, on_stripe, &sc, &gpu_secs, err, sizeof err);
         cgh_dataset_free(ds);
-        if (rc) die(std::string("GPU: ") + err);
+        if (rc) die(std::string("CUDA: ") + err);  // main.rs:460
         snprintf(msg, sizeof msg, "Computed %zu x %zu matrix on MFMA in %.2fs (GPU pipeline %.2fs, rare pairs on GPU)",
                  n, n, now() - t, gpu_secs);
         log(msg);

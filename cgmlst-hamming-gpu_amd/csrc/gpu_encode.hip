@@ -42,14 +42,6 @@ void set_err(char* err, int errlen, const char* fmt, ...) {
 
 size_t up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
-#define HE(x)                                                                  \
-    do {                                                                       \
-        hipError_t e_ = (x);                                                   \
-        if (e_ != hipSuccess) {                                                \
-            set_err(err, errlen, "%s: %s", #x, hipGetErrorString(e_));         \
-            goto fail;                                                         \
-        }                                                                      \
-    } while (0)
 
 // keys[k n + i] = calls[i L + k], vals[k n + i] = i, through a 32 x 32 LDS
 // tile so that both the reads and the writes are coalesced.
@@ -187,52 +179,52 @@ extern "C" int cgh_dataset_encode(int device, const uint64_t* calls, int n, int 
     ds.device = device;
     ds.n = n;
     ds.L = L;
-    HE(hipSetDevice(device));
+    HIP_TRY(hipSetDevice(device));
     // 1. locus-major keys with row indices, sorted per locus
-    HE(hipMalloc(&dcalls, total * 8));
-    HE(hipMemcpy(dcalls, calls, total * 8, hipMemcpyHostToDevice));
-    HE(hipMalloc(&dkeys, total * 8));
-    HE(hipMalloc(&dvals, total * 4));
+    HIP_TRY(hipMalloc(&dcalls, total * 8));
+    HIP_TRY(hipMemcpy(dcalls, calls, total * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&dkeys, total * 8));
+    HIP_TRY(hipMalloc(&dvals, total * 4));
     hipLaunchKernelGGL(transpose_calls, dim3((unsigned)((L + 31) / 32), (unsigned)((n + 31) / 32)), dim3(256), 0, 0,
                        dcalls, n, L, dkeys, dvals);
-    HE(hipGetLastError());
-    HE(hipFree(dcalls));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipFree(dcalls));
     dcalls = NULL;
-    HE(hipMalloc(&dkeys2, total * 8));
-    HE(hipMalloc(&dvals2, total * 4));
+    HIP_TRY(hipMalloc(&dkeys2, total * 8));
+    HIP_TRY(hipMalloc(&dvals2, total * 4));
     off.resize((size_t)L + 1);
     for (int k = 0; k <= L; k++) off[k] = k * n;
-    HE(hipMalloc(&doff, off.size() * sizeof(int)));
-    HE(hipMemcpy(doff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
-    HE(hipcub::DeviceSegmentedRadixSort::SortPairs(NULL, temp_bytes, dkeys, dkeys2, dvals, dvals2, (int)total, L, doff,
+    HIP_TRY(hipMalloc(&doff, off.size() * sizeof(int)));
+    HIP_TRY(hipMemcpy(doff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipcub::DeviceSegmentedRadixSort::SortPairs(NULL, temp_bytes, dkeys, dkeys2, dvals, dvals2, (int)total, L, doff,
                                                    doff + 1));
-    HE(hipMalloc(&dtemp, std::max<size_t>(temp_bytes, 256)));
-    HE(hipcub::DeviceSegmentedRadixSort::SortPairs(dtemp, temp_bytes, dkeys, dkeys2, dvals, dvals2, (int)total, L,
+    HIP_TRY(hipMalloc(&dtemp, std::max<size_t>(temp_bytes, 256)));
+    HIP_TRY(hipcub::DeviceSegmentedRadixSort::SortPairs(dtemp, temp_bytes, dkeys, dkeys2, dvals, dvals2, (int)total, L,
                                                    doff, doff + 1));
-    HE(hipFree(dkeys));
+    HIP_TRY(hipFree(dkeys));
     dkeys = NULL;
-    HE(hipFree(dvals));
+    HIP_TRY(hipFree(dvals));
     dvals = NULL;
-    HE(hipFree(dtemp));
+    HIP_TRY(hipFree(dtemp));
     dtemp = NULL;
     // 2. dense ranks: scan of "new distinct id" flags
-    HE(hipMalloc(&dflag, total * 4));
-    HE(hipMalloc(&dscan, total * 4));
+    HIP_TRY(hipMalloc(&dflag, total * 4));
+    HIP_TRY(hipMalloc(&dscan, total * 4));
     hipLaunchKernelGGL(distinct_flags, dim3(blocks(total)), dim3(256), 0, 0, dkeys2, n, total, dflag);
-    HE(hipGetLastError());
+    HIP_TRY(hipGetLastError());
     temp_bytes = 0;
-    HE(hipcub::DeviceScan::InclusiveSum(NULL, temp_bytes, dflag, dscan, (int)total));
-    HE(hipMalloc(&dtemp, std::max<size_t>(temp_bytes, 256)));
-    HE(hipcub::DeviceScan::InclusiveSum(dtemp, temp_bytes, dflag, dscan, (int)total));
-    HE(hipFree(dtemp));
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(NULL, temp_bytes, dflag, dscan, (int)total));
+    HIP_TRY(hipMalloc(&dtemp, std::max<size_t>(temp_bytes, 256)));
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(dtemp, temp_bytes, dflag, dscan, (int)total));
+    HIP_TRY(hipFree(dtemp));
     dtemp = NULL;
-    HE(hipFree(dflag));
+    HIP_TRY(hipFree(dflag));
     dflag = NULL;
-    HE(hipMalloc(&ddist, (size_t)L * 4));
+    HIP_TRY(hipMalloc(&ddist, (size_t)L * 4));
     hipLaunchKernelGGL(locus_distinct, dim3(blocks((size_t)L)), dim3(256), 0, 0, dscan, n, L, ddist);
-    HE(hipGetLastError());
+    HIP_TRY(hipGetLastError());
     distinct.resize((size_t)L);
-    HE(hipMemcpy(distinct.data(), ddist, (size_t)L * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(distinct.data(), ddist, (size_t)L * 4, hipMemcpyDeviceToHost));
     for (int k = 0; k < L; k++)
         if (distinct[k] > 65535) {
             set_err(err, errlen, "locus %d has %u distinct alleles (limit 65535)", k, distinct[k]);
@@ -245,16 +237,16 @@ extern "C" int cgh_dataset_encode(int device, const uint64_t* calls, int n, int 
         ncells += (size_t)distinct[k] + 1;
     }
     colstart[L] = (uint32_t)ncells;
-    HE(hipMalloc(&dcs, colstart.size() * 4));
-    HE(hipMemcpy(dcs, colstart.data(), colstart.size() * 4, hipMemcpyHostToDevice));
-    HE(hipMalloc(&dcnt, ncells * 4));
-    HE(hipMemset(dcnt, 0, ncells * 4));
-    HE(hipMalloc(&dX, total * 2));
+    HIP_TRY(hipMalloc(&dcs, colstart.size() * 4));
+    HIP_TRY(hipMemcpy(dcs, colstart.data(), colstart.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&dcnt, ncells * 4));
+    HIP_TRY(hipMemset(dcnt, 0, ncells * 4));
+    HIP_TRY(hipMalloc(&dX, total * 2));
     hipLaunchKernelGGL(scatter_ranks, dim3(blocks(total)), dim3(256), 0, 0, dkeys2, dvals2, dscan, n, L, dcs, dX,
                        dcnt);
-    HE(hipGetLastError());
+    HIP_TRY(hipGetLastError());
     cnt.resize(ncells);
-    HE(hipMemcpy(cnt.data(), dcnt, ncells * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cnt.data(), dcnt, ncells * 4, hipMemcpyDeviceToHost));
     // 4. tau and the tables (host: a few bytes per distinct allele)
     if (tau == 0) {
         std::vector<uint32_t> c1;
@@ -302,13 +294,13 @@ extern "C" int cgh_dataset_encode(int device, const uint64_t* calls, int n, int 
     ds.n_pad = up((size_t)n, 256);
     ds.kw1 = (size_t)cghi::device_words(up(std::max<size_t>(D, 1), 256) / 32);
     ds.kw2 = (size_t)cghi::device_words(up((size_t)L, 256) / 32);
-    HE(hipMalloc(&dcm, ncells * 4));
-    HE(hipMemcpy(dcm, colmap.data(), ncells * 4, hipMemcpyHostToDevice));
-    HE(hipMalloc(&ds.dO, ds.n_pad * ds.kw1 * 4));
-    HE(hipMalloc(&ds.dP, ds.n_pad * ds.kw2 * 4));
-    HE(hipMemset(ds.dO, 0, ds.n_pad * ds.kw1 * 4));
-    HE(hipMemset(ds.dP, 0, ds.n_pad * ds.kw2 * 4));
-    HE(cghi::launch_build_planes(dX, n, L, dcs, dcm, ds.dO, ds.dP, (int)ds.n_pad, 0));
+    HIP_TRY(hipMalloc(&dcm, ncells * 4));
+    HIP_TRY(hipMemcpy(dcm, colmap.data(), ncells * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&ds.dO, ds.n_pad * ds.kw1 * 4));
+    HIP_TRY(hipMalloc(&ds.dP, ds.n_pad * ds.kw2 * 4));
+    HIP_TRY(hipMemset(ds.dO, 0, ds.n_pad * ds.kw1 * 4));
+    HIP_TRY(hipMemset(ds.dP, 0, ds.n_pad * ds.kw2 * 4));
+    HIP_TRY(cghi::launch_build_planes(dX, n, L, dcs, dcm, ds.dO, ds.dP, (int)ds.n_pad, 0));
     // 6. rare member lists and their row index
     ds.n_groups = groups;
     ds.n_members = members;
@@ -316,43 +308,43 @@ extern "C" int cgh_dataset_encode(int device, const uint64_t* calls, int n, int 
     ds.n_rare = rare;
     ds.m16 = n <= 65536;
     if (groups) {
-        HE(hipMalloc(&drun, ncells * 4));
-        HE(hipMemcpy(drun, runstart.data(), ncells * 4, hipMemcpyHostToDevice));
-        HE(hipMalloc(&dgid, ncells * 8));
-        HE(hipMemcpy(dgid, gid.data(), ncells * 8, hipMemcpyHostToDevice));
-        HE(hipMalloc(&dgs, gstart.size() * 4));
-        HE(hipMemcpy(dgs, gstart.data(), gstart.size() * 4, hipMemcpyHostToDevice));
-        HE(hipMalloc(&ds.dgmem, members * (ds.m16 ? 2 : 4)));
-        HE(hipMalloc(&drc, ((size_t)n + 1) * 4));
-        HE(hipMemset(drc, 0, ((size_t)n + 1) * 4));
+        HIP_TRY(hipMalloc(&drun, ncells * 4));
+        HIP_TRY(hipMemcpy(drun, runstart.data(), ncells * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&dgid, ncells * 8));
+        HIP_TRY(hipMemcpy(dgid, gid.data(), ncells * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&dgs, gstart.size() * 4));
+        HIP_TRY(hipMemcpy(dgs, gstart.data(), gstart.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&ds.dgmem, members * (ds.m16 ? 2 : 4)));
+        HIP_TRY(hipMalloc(&drc, ((size_t)n + 1) * 4));
+        HIP_TRY(hipMemset(drc, 0, ((size_t)n + 1) * 4));
         if (ds.m16)
             hipLaunchKernelGGL(rare_members<uint16_t>, dim3(blocks(total)), dim3(256), 0, 0, dkeys2, dvals2, dscan, n,
                                L, dcs, dgid, drun, dgs, (uint16_t*)ds.dgmem, drc);
         else
             hipLaunchKernelGGL(rare_members<uint32_t>, dim3(blocks(total)), dim3(256), 0, 0, dkeys2, dvals2, dscan, n,
                                L, dcs, dgid, drun, dgs, (uint32_t*)ds.dgmem, drc);
-        HE(hipGetLastError());
-        HE(hipMalloc(&ds.drstart, ((size_t)n + 1) * 4));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMalloc(&ds.drstart, ((size_t)n + 1) * 4));
         temp_bytes = 0;
-        HE(hipcub::DeviceScan::ExclusiveSum(NULL, temp_bytes, drc, ds.drstart, n + 1));
-        HE(hipMalloc(&dtemp, std::max<size_t>(temp_bytes, 256)));
-        HE(hipcub::DeviceScan::ExclusiveSum(dtemp, temp_bytes, drc, ds.drstart, n + 1));
-        HE(hipMalloc(&dcur, ((size_t)n + 1) * 4));
-        HE(hipMemcpy(dcur, ds.drstart, ((size_t)n + 1) * 4, hipMemcpyDeviceToDevice));
-        HE(hipMalloc(&dgend, members * 4));
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(NULL, temp_bytes, drc, ds.drstart, n + 1));
+        HIP_TRY(hipMalloc(&dtemp, std::max<size_t>(temp_bytes, 256)));
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(dtemp, temp_bytes, drc, ds.drstart, n + 1));
+        HIP_TRY(hipMalloc(&dcur, ((size_t)n + 1) * 4));
+        HIP_TRY(hipMemcpy(dcur, ds.drstart, ((size_t)n + 1) * 4, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMalloc(&dgend, members * 4));
         hipLaunchKernelGGL(group_ends, dim3(blocks(groups)), dim3(256), 0, 0, dgs, groups, dgend);
-        HE(hipGetLastError());
-        HE(hipMalloc(&ds.drentry, std::max<size_t>(rentries, 1) * 8));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMalloc(&ds.drentry, std::max<size_t>(rentries, 1) * 8));
         if (ds.m16)
             hipLaunchKernelGGL(row_entries<uint16_t>, dim3(blocks(members)), dim3(256), 0, 0,
                                (const uint16_t*)ds.dgmem, dgend, members, dcur, ds.drentry);
         else
             hipLaunchKernelGGL(row_entries<uint32_t>, dim3(blocks(members)), dim3(256), 0, 0,
                                (const uint32_t*)ds.dgmem, dgend, members, dcur, ds.drentry);
-        HE(hipGetLastError());
+        HIP_TRY(hipGetLastError());
     }
-    if (X_out) HE(hipMemcpy(X_out, dX, total * 2, hipMemcpyDeviceToHost));
-    HE(hipDeviceSynchronize());
+    if (X_out) HIP_TRY(hipMemcpy(X_out, dX, total * 2, hipMemcpyDeviceToHost));
+    HIP_TRY(hipDeviceSynchronize());
     *out = cghi::dataset_adopt(ds);
     if (!*out) {
         set_err(err, errlen, "out of host memory");

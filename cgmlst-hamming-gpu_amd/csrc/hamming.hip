@@ -699,25 +699,6 @@ valu_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* 
 
 __global__ void reset_counter(int* c) { *c = 0; }
 
-// Subtract one from the u16 cell at element offset off of a 4-byte aligned
-// u16 array.  The cell never goes below the value it ends with, so the
-// half-word arithmetic inside the 32-bit atomic never borrows.
-__device__ __forceinline__ void sub_one_u16(uint16_t* base, size_t off) {
-    uint32_t* wp = reinterpret_cast<uint32_t*>(base) + (off >> 1);
-    atomicSub(wp, (off & 1) ? 0x10000u : 1u);
-}
-
-// Reference CSR form (tch.cu:36-43): one wavefront per stripe row, lanes
-// stride over the row's partners j > i.
-__global__ void __launch_bounds__(256)
-sub_rare_csr(uint16_t* out, int w, int r0, int rows, const uint32_t* start, const uint32_t* js) {
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (r >= rows) return;
-    const int i = r0 + r;
-    const size_t rowoff = (size_t)r * w - r0;
-    for (uint32_t p = start[i] + lane; p < start[i + 1]; p += 64) sub_one_u16(out, rowoff + js[p]);
-}
-
 // Rare pairs, row-local: a workgroup takes items (row i and a chunk of RCW
 // columns, or a group of short rows) one after the other.  An item's cells
 // are staged in LDS with 16-byte loads, every rare partner j of its rows is
@@ -752,7 +733,31 @@ struct RareJob {
     // G = 1: one row per workgroup, in `chunks` column chunks.
     int G, gps;
     int rsg;                   // lanes per range (a power of two, 4 .. 32)
+    // Members ascending within each range (group member lists).  A range
+    // in arbitrary order (a caller's per-row partner list) is walked whole in
+    // every column chunk, members outside the chunk masked out.
+    int sorted;
 };
+
+// Rare job over the stripe layout `out` of rows r0.. (symmetric form).
+RareJob rare_job(uint16_t* out, int n, int r0, int S, const uint32_t* rstart, const uint64_t* rentry,
+                 const void* gmem, int m16, int cap, int rsg) {
+    RareJob j{};
+    j.out = out;
+    j.n = n;
+    j.r0 = r0;
+    j.S = S;
+    j.chunks = 1;
+    j.rstart = rstart;
+    j.rentry = rentry;
+    j.gmem = gmem;
+    j.m16 = m16;
+    j.cap = cap;
+    j.G = 1;
+    j.rsg = rsg;
+    j.sorted = 1;
+    return j;
+}
 
 struct RareShared {
     uint4 cells4[RCW / 8 + 1];
@@ -840,8 +845,9 @@ __device__ __forceinline__ void rare_stage_land(const RareItem& it, const v4i (&
 // threads at once into LDS; then each rsg-lane subgroup takes ranges from
 // LDS, its lanes reading consecutive members (RUN loads in flight per lane)
 // and applying them with LDS atomics.  The first barrier also orders the
-// staging stores before the atomics.
-template <typename M>
+// staging stores before the atomics.  MASK: ranges in arbitrary order (not
+// clipped); a member outside the item's columns [j0, j1) is skipped.
+template <typename M, bool MASK>
 __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restrict__ gmem, const RareItem& it,
                                             RareShared& sh) {
     const uint32_t* __restrict__ rstart = jb.rstart;
@@ -858,7 +864,7 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
             if (tid < nb) {
                 const uint64_t r = rentry[eb + tid];
                 uint32_t b = (uint32_t)r, end = (uint32_t)(r >> 32);
-                if (jb.chunks > 1) {  // clip the ascending range to [j0, j1)
+                if (!MASK && jb.chunks > 1) {  // clip the ascending range to [j0, j1)
                     uint32_t lo = b, hi = end;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) >> 1;
@@ -926,7 +932,8 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
                 if (cp0 < cend) {
 #pragma unroll
                     for (int u = 0; u < RUN; u++) {
-                        const bool in = cp0 + sl + u * rsg < cend;
+                        const bool in = cp0 + sl + u * rsg < cend &&
+                                        (!MASK || (ja[u] >= (uint32_t)it.j0 && ja[u] < (uint32_t)it.j1));
                         const uint32_t o = bo + ja[u];
                         atomicSub(&cells[in ? (o >> 1) : own], in ? ((o & 1) ? 0x10000u : 1u) : 0u);
                     }
@@ -949,7 +956,8 @@ __device__ __forceinline__ void rare_write_back(const RareJob& jb, const RareIte
     const int tid = threadIdx.x;
     if (jb.cap > 0) {  // main.rs:251-253 on the final values
         uint16_t* c16 = reinterpret_cast<uint16_t*>(sh.cells4);
-        for (int k = tid; k < it.nq * 8; k += RNT) c16[k] = min(c16[k], (uint16_t)jb.cap);
+        const uint16_t cap = (uint16_t)min(jb.cap, 0xffff);
+        for (int k = tid; k < it.nq * 8; k += RNT) c16[k] = min(c16[k], cap);
         __syncthreads();
     }
     uint16_t* __restrict__ out = jb.out;
@@ -970,7 +978,7 @@ __device__ __forceinline__ void rare_write_back(const RareJob& jb, const RareIte
 // workgroup.  The next item's staging loads are issued before the current
 // item is written back, so their latency is spent under the write-back
 // instead of in front of the next item's atomics.
-template <typename M>
+template <typename M, bool MASK>
 __device__ __forceinline__ void rare_items(const RareJob& jb, const M* __restrict__ gmem, long long b0,
                                            long long nitems, long long step, RareShared& sh) {
     RareItem cur, nxt;
@@ -982,7 +990,7 @@ __device__ __forceinline__ void rare_items(const RareJob& jb, const M* __restric
     rare_stage_issue(jb, cur, v, SEQ);
     while (true) {
         rare_stage_land(cur, v, sh, SEQ);
-        rare_ranges(jb, gmem, cur, sh);
+        rare_ranges<M, MASK>(jb, gmem, cur, sh);
         long long nb = b + step;
         while (nb < nitems && !rare_item(jb, nb, nxt)) nb += step;
         if (nb < nitems) rare_stage_issue(jb, nxt, v, SEQ);
@@ -996,16 +1004,19 @@ __device__ __forceinline__ void rare_items(const RareJob& jb, const M* __restric
 
 // Member lists are read as u16 when every row index fits: half the bytes of
 // the pass's dominant (L2-missing) reads.
+template <bool MASK>
 __device__ __forceinline__ void rare_items_any(const RareJob& jb, long long b0, long long nitems, long long step,
                                                RareShared& sh) {
-    if (jb.m16) rare_items(jb, static_cast<const uint16_t*>(jb.gmem), b0, nitems, step, sh);
-    else rare_items(jb, static_cast<const uint32_t*>(jb.gmem), b0, nitems, step, sh);
+    if (jb.m16) rare_items<uint16_t, MASK>(jb, static_cast<const uint16_t*>(jb.gmem), b0, nitems, step, sh);
+    else rare_items<uint32_t, MASK>(jb, static_cast<const uint32_t*>(jb.gmem), b0, nitems, step, sh);
 }
 
 // One job, a persistent grid: workgroup w takes items w, w + grid, ...
+// MASK: the job's ranges are unsorted (RareJob::sorted == 0).
+template <bool MASK>
 __global__ void __launch_bounds__(RNT) sub_rare_rows(RareJob jb, long long nitems) {
     __shared__ RareShared sh;
-    rare_items_any(jb, blockIdx.x, nitems, gridDim.x, sh);
+    rare_items_any<MASK>(jb, blockIdx.x, nitems, gridDim.x, sh);
 }
 
 
@@ -1042,6 +1053,13 @@ long long rare_layout(RareJob& jb, int rows) {
     return (long long)rows * jb.chunks;
 }
 
+// One persistent rare launch of job jb over `items` work items.
+hipError_t launch_rare(const RareJob& jb, long long items, hipStream_t st) {
+    if (jb.sorted) hipLaunchKernelGGL(sub_rare_rows<false>, dim3(rare_grid(items)), dim3(RNT), 0, st, jb, items);
+    else hipLaunchKernelGGL(sub_rare_rows<true>, dim3(rare_grid(items)), dim3(RNT), 0, st, jb, items);
+    return hipGetLastError();
+}
+
 void set_err(char* err, int errlen, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 void set_err(char* err, int errlen, const char* fmt, ...) {
     if (!err || errlen <= 0) return;
@@ -1051,14 +1069,6 @@ void set_err(char* err, int errlen, const char* fmt, ...) {
     va_end(ap);
 }
 
-#define HK(x)                                                                           \
-    do {                                                                                \
-        hipError_t e_ = (x);                                                            \
-        if (e_ != hipSuccess) {                                                         \
-            set_err(err, errlen, "%s: %s", #x, hipGetErrorString(e_));                  \
-            goto fail;                                                                  \
-        }                                                                               \
-    } while (0)
 
 // Tile shapes: 256 (4 wavefronts of 128 x 128) for large sets, 128 (4
 // wavefronts of 64 x 64) when the 256 grid would leave most CUs idle.
@@ -1226,100 +1236,6 @@ extern "C" size_t tch_free_vram(int device) {
     return f;
 }
 
-extern "C" int tch_run(int device, int n, size_t n_pad, const uint8_t* O, size_t K1, const uint8_t* P,
-                       size_t K2, const uint32_t* start, const uint32_t* js, size_t n_rare, int S, tch_cb cb,
-                       void* ctx, double* gpu_secs, int* rare_on_gpu, char* err, int errlen) {
-    uint32_t *dO = NULL, *dP = NULL, *dStart = NULL, *dJ = NULL;
-    uint16_t *dU[2] = {NULL, NULL}, *hU[2] = {NULL, NULL};
-    hipStream_t st[2] = {NULL, NULL};
-    hipEvent_t done[2] = {NULL, NULL}, t0 = NULL, t1 = NULL;
-    size_t freeb = 0, totb = 0;
-    bool rare_gpu = false;
-    int rc = 1, prev = -1;
-    float ms = 0.f;
-    const int tile = pick_tile(n);
-    const size_t stripe_elems = up((size_t)S * n_pad, 2);
-
-    if (n < 1 || S < 8 || S % 8 || K1 % 1024 || K2 % 1024 || n_pad % 256 || n_pad < (size_t)n) {
-        set_err(err, errlen, "tch_run: bad shape (n=%d n_pad=%zu K1=%zu K2=%zu S=%d)", n, n_pad, K1, K2, S);
-        return 1;
-    }
-    HK(hipSetDevice(device));
-    // the caller's row-major bit matrices -> device planes
-    HK(upload_planes(O, (int)n_pad, (int)(K1 / 32), kc_words(K1 / 32), &dO));
-    HK(upload_planes(P, (int)n_pad, (int)(K2 / 32), kc_words(K2 / 32), &dP));
-    HK(hipMemGetInfo(&freeb, &totb));
-    {
-        const char* env = getenv("CGH_FREE_BYTES");
-        if (env && *env) freeb = std::min(freeb, (size_t)strtoull(env, NULL, 10));
-    }
-    // same capacity rule as tch.cu:105, with this pipeline's u16-only stripes
-    rare_gpu = (n_rare + n + 1) * 4 + stripe_elems * 2 * 2 + (64u << 20) < freeb;
-    if (rare_gpu) {
-        HK(hipMalloc(&dStart, (size_t)(n + 1) * 4));
-        HK(hipMalloc(&dJ, std::max<size_t>(n_rare, 1) * 4));
-        HK(hipMemcpy(dStart, start, (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
-        if (n_rare) HK(hipMemcpy(dJ, js, n_rare * 4, hipMemcpyHostToDevice));
-    }
-    if (rare_on_gpu) *rare_on_gpu = rare_gpu;
-    for (int b = 0; b < 2; b++) {
-        HK(hipMalloc(&dU[b], stripe_elems * 2));
-        HK(hipHostMalloc(&hU[b], stripe_elems * 2, hipHostMallocDefault));
-        HK(hipStreamCreateWithFlags(&st[b], hipStreamNonBlocking));
-        HK(hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
-    }
-    HK(hipEventCreate(&t0));
-    HK(hipEventCreate(&t1));
-    HK(hipEventRecord(t0, st[0]));
-    for (int r0 = 0;; r0 += S) {
-        if (r0 < n) {
-            const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
-            const size_t w = (size_t)(n - r0);
-            HK(launch_tiles(resolve_kernel(CGH_KERNEL_AUTO), tile, dO, kc_words(K1 / 32), dP, kc_words(K2 / 32),
-                            (int)n_pad, n, r0, S, 1, dU[b], st[b]));
-            if (rare_gpu && n_rare) {
-                hipLaunchKernelGGL(sub_rare_csr, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st[b], dU[b],
-                                   (int)w, r0, rows, dStart, dJ);
-                HK(hipGetLastError());
-            }
-            HK(hipMemcpyAsync(hU[b], dU[b], (size_t)rows * w * 2, hipMemcpyDeviceToHost, st[b]));
-            HK(hipEventRecord(done[b], st[b]));
-        }
-        if (prev >= 0) {  // consume the previous stripe while this one runs
-            const int rows = std::min(S, n - prev), b = (prev / S) & 1;
-            const size_t w = (size_t)(n - prev);
-            HK(hipEventSynchronize(done[b]));
-            if (!rare_gpu)
-                for (int r = 0; r < rows; r++) {
-                    const int i = prev + r;
-                    for (uint32_t p = start[i]; p < start[i + 1]; p++) hU[b][(size_t)r * w + (js[p] - prev)] -= 1;
-                }
-            cb(ctx, prev, rows, w, hU[b]);
-        }
-        if (r0 >= n) break;
-        prev = r0;
-    }
-    HK(hipEventRecord(t1, st[0]));
-    HK(hipEventSynchronize(t1));
-    HK(hipEventElapsedTime(&ms, t0, t1));
-    if (gpu_secs) *gpu_secs = ms / 1e3;
-    rc = 0;
-fail:
-    for (int b = 0; b < 2; b++) {
-        if (dU[b]) (void)hipFree(dU[b]);
-        if (hU[b]) (void)hipHostFree(hU[b]);
-        if (st[b]) (void)hipStreamDestroy(st[b]);
-        if (done[b]) (void)hipEventDestroy(done[b]);
-    }
-    if (t0) (void)hipEventDestroy(t0);
-    if (t1) (void)hipEventDestroy(t1);
-    if (dO) (void)hipFree(dO);
-    if (dP) (void)hipFree(dP);
-    if (dStart) (void)hipFree(dStart);
-    if (dJ) (void)hipFree(dJ);
-    return rc;
-}
-
 // ---------------------------------------------------------------------------
 // Device-resident dataset
 // ---------------------------------------------------------------------------
@@ -1379,15 +1295,15 @@ int dataset_meta(int device, const cgh_encoding* enc, cgh_dataset** out, char* e
         free(ds);
         return 1;
     }
-    HK(hipSetDevice(device));
+    HIP_TRY(hipSetDevice(device));
     if (enc->n_groups) {
         ds->m16 = enc->n <= 65536;
-        HK(upload_members(enc->gmem, enc->n_members, ds->m16, &ds->dgmem));
-        HK(hipMalloc(&ds->drstart, ((size_t)enc->n + 1) * 4));
-        HK(hipMalloc(&ds->drentry, std::max<size_t>(enc->n_rentries, 1) * 8));
-        HK(hipMemcpy(ds->drstart, enc->rstart, ((size_t)enc->n + 1) * 4, hipMemcpyHostToDevice));
+        HIP_TRY(upload_members(enc->gmem, enc->n_members, ds->m16, &ds->dgmem));
+        HIP_TRY(hipMalloc(&ds->drstart, ((size_t)enc->n + 1) * 4));
+        HIP_TRY(hipMalloc(&ds->drentry, std::max<size_t>(enc->n_rentries, 1) * 8));
+        HIP_TRY(hipMemcpy(ds->drstart, enc->rstart, ((size_t)enc->n + 1) * 4, hipMemcpyHostToDevice));
         if (enc->n_rentries)
-            HK(hipMemcpy(ds->drentry, enc->rentry, enc->n_rentries * 8, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(ds->drentry, enc->rentry, enc->n_rentries * 8, hipMemcpyHostToDevice));
     }
     *out = ds;
     return 0;
@@ -1402,8 +1318,8 @@ extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_datas
     cgh_dataset* ds = NULL;
     *out = NULL;
     if (dataset_meta(device, enc, &ds, err, errlen)) return 1;
-    HK(upload_planes(enc->O, (int)enc->n_pad, (int)enc->kw1, (int)ds->kw1, &ds->dO));
-    HK(upload_planes(enc->P, (int)enc->n_pad, (int)enc->kw2, (int)ds->kw2, &ds->dP));
+    HIP_TRY(upload_planes(enc->O, (int)enc->n_pad, (int)enc->kw1, (int)ds->kw1, &ds->dO));
+    HIP_TRY(upload_planes(enc->P, (int)enc->n_pad, (int)enc->kw2, (int)ds->kw2, &ds->dP));
     *out = ds;
     return 0;
 fail:
@@ -1433,22 +1349,22 @@ extern "C" int cgh_dataset_build(int device, const uint16_t* X, int n, int L, in
     colstart.resize((size_t)L + 1);
     colmap.resize(need);
     cgh_column_table(X, n, L, tau, colstart.data(), colmap.data(), need);
-    HK(hipMalloc(&ds->dO, ds->n_pad * ds->kw1 * 4));
-    HK(hipMalloc(&ds->dP, ds->n_pad * ds->kw2 * 4));
-    HK(hipMemset(ds->dO, 0, ds->n_pad * ds->kw1 * 4));
-    HK(hipMemset(ds->dP, 0, ds->n_pad * ds->kw2 * 4));
-    HK(hipMalloc(&dX, (size_t)n * L * 2));
-    HK(hipMalloc(&dcs, colstart.size() * 4));
-    HK(hipMalloc(&dcm, std::max<size_t>(need, 1) * 4));
-    HK(hipMemcpy(dX, X, (size_t)n * L * 2, hipMemcpyHostToDevice));
-    HK(hipMemcpy(dcs, colstart.data(), colstart.size() * 4, hipMemcpyHostToDevice));
-    HK(hipMemcpy(dcm, colmap.data(), need * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&ds->dO, ds->n_pad * ds->kw1 * 4));
+    HIP_TRY(hipMalloc(&ds->dP, ds->n_pad * ds->kw2 * 4));
+    HIP_TRY(hipMemset(ds->dO, 0, ds->n_pad * ds->kw1 * 4));
+    HIP_TRY(hipMemset(ds->dP, 0, ds->n_pad * ds->kw2 * 4));
+    HIP_TRY(hipMalloc(&dX, (size_t)n * L * 2));
+    HIP_TRY(hipMalloc(&dcs, colstart.size() * 4));
+    HIP_TRY(hipMalloc(&dcm, std::max<size_t>(need, 1) * 4));
+    HIP_TRY(hipMemcpy(dX, X, (size_t)n * L * 2, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dcs, colstart.data(), colstart.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dcm, colmap.data(), need * 4, hipMemcpyHostToDevice));
     {
         const int lb = (L + 255) / 256;
         hipLaunchKernelGGL(build_planes, dim3((unsigned)((size_t)n * lb)), dim3(256), 0, 0, dX, L, lb, dcs, dcm,
                            ds->dO, ds->dP, (int)ds->n_pad);
-        HK(hipGetLastError());
-        HK(hipDeviceSynchronize());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
     }
     (void)hipFree(dX);
     (void)hipFree(dcs);
@@ -1466,6 +1382,12 @@ fail:
 }
 
 namespace cghi {
+
+bool hip_failed(hipError_t e, const char* call, char* err, int errlen) {
+    if (e == hipSuccess) return false;
+    set_err(err, errlen, "HIP error '%s' in %s", hipGetErrorString(e), call);
+    return true;
+}
 
 cgh_dataset* dataset_adopt(const DatasetParts& p) {
     cgh_dataset* ds = (cgh_dataset*)calloc(1, sizeof(cgh_dataset));
@@ -1505,7 +1427,8 @@ hipError_t launch_build_planes(const uint16_t* dX, int n, int L, const uint32_t*
 
 extern "C" int cgh_dataset_set_cap(cgh_dataset* ds, int cap) {
     if (cap < 0) return 1;
-    ds->cap = cap;
+    // every distance is <= L <= 65,535: a cap from 65,535 up changes nothing
+    ds->cap = cap >= 65535 ? 0 : cap;
     return 0;
 }
 
@@ -1558,22 +1481,22 @@ extern "C" int cgh_dataset_autotune(cgh_dataset* ds, int S, int* kernel_out, int
         set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
         return 1;
     }
-    HK(hipSetDevice(ds->device));
-    HK(hipMalloc(&buf, up(stripe_off((size_t)n, (size_t)S, (size_t)ns), 8) * 2 + 16));
-    HK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    HK(hipEventCreate(&e0));
-    HK(hipEventCreate(&e1));
+    HIP_TRY(hipSetDevice(ds->device));
+    HIP_TRY(hipMalloc(&buf, up(stripe_off((size_t)n, (size_t)S, (size_t)ns), 8) * 2 + 16));
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
     for (int k : kernels)
         for (int t : tiles) {
             float ms = 0.f;
-            HK(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf, st));
-            HK(hipEventRecord(e0, st));
+            HIP_TRY(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf, st));
+            HIP_TRY(hipEventRecord(e0, st));
             for (int rep = 0; rep < 3; rep++)
-                HK(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf,
+                HIP_TRY(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf,
                                 st));
-            HK(hipEventRecord(e1, st));
-            HK(hipEventSynchronize(e1));
-            HK(hipEventElapsedTime(&ms, e0, e1));
+            HIP_TRY(hipEventRecord(e1, st));
+            HIP_TRY(hipEventSynchronize(e1));
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
             if (ms / 3 < best) {
                 best = ms / 3;
                 bk = k;
@@ -1614,16 +1537,15 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
         set_err(err, errlen, "output buffer must be 4-byte aligned");
         return 1;
     }
-    HK(hipSetDevice(ds->device));
+    HIP_TRY(hipSetDevice(ds->device));
     if (phases & CGH_PHASE_PRODUCT)
-        HK(launch_tiles(ds_kernel(ds, kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
+        HIP_TRY(launch_tiles(ds_kernel(ds, kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
                          S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
-        RareJob jb = {d_out, n, 0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, ds->m16, ds->cap, 1, 0,
-                      lanes_per_range(ds->n_rare, ds->n_rentries)};
+        RareJob jb = rare_job(d_out, n, 0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap,
+                              lanes_per_range(ds->n_rare, ds->n_rentries));
         const long long items = rare_layout(jb, n);
-        hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(items)), dim3(RNT), 0, st, jb, items);
-        HK(hipGetLastError());
+        HIP_TRY(launch_rare(jb, items, st));
     }
     return 0;
 fail:
@@ -1722,19 +1644,19 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
                 for (int J = 0; J * tl < w; J++) lists[k].push_back({J >= I ? cost : 0, make_int4(d, s, I, J)});
         }
         if (ds->n_rare && ds->n > 1) {
-            RareJob r = {NULL, ds->n, 0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, ds->m16, ds->cap, 1, 0,
-                         lanes_per_range(ds->n_rare, ds->n_rentries)};
+            RareJob r = rare_job(NULL, ds->n, 0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap,
+                                 lanes_per_range(ds->n_rare, ds->n_rentries));
             b->rare_items.push_back(rare_layout(r, ds->n));
             b->rare.push_back(r);
             b->rare_ds.push_back(d);
         }
     }
-    HK(hipSetDevice(device));
-    HK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
-    HK(hipMalloc(&b->djobs, sizeof(TileJob) * (size_t)count));
-    HK(hipMemcpy(b->djobs, jobs.data(), sizeof(TileJob) * (size_t)count, hipMemcpyHostToDevice));
-    HK(hipMalloc(&b->doffs, sizeof(size_t) * (size_t)count));
-    HK(hipMemcpy(b->doffs, b->offs.data(), sizeof(size_t) * (size_t)count, hipMemcpyHostToDevice));
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
+    HIP_TRY(hipMalloc(&b->djobs, sizeof(TileJob) * (size_t)count));
+    HIP_TRY(hipMemcpy(b->djobs, jobs.data(), sizeof(TileJob) * (size_t)count, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&b->doffs, sizeof(size_t) * (size_t)count));
+    HIP_TRY(hipMemcpy(b->doffs, b->offs.data(), sizeof(size_t) * (size_t)count, hipMemcpyHostToDevice));
     for (int k = 0; k < 2; k++) {
         // longest tiles first; below-diagonal (zero-fill) tiles last
         std::stable_sort(lists[k].begin(), lists[k].end(),
@@ -1745,10 +1667,10 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         for (auto& e : lists[k]) t.push_back(e.second);
         b->ntiles[k] = (int)t.size();
         if (t.empty()) continue;
-        HK(hipMalloc(&b->dtiles[k], sizeof(int4) * t.size()));
-        HK(hipMemcpy(b->dtiles[k], t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&b->dtiles[k], sizeof(int4) * t.size()));
+        HIP_TRY(hipMemcpy(b->dtiles[k], t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
     }
-    HK(hipMalloc(&b->dcount, 2 * sizeof(int)));
+    HIP_TRY(hipMalloc(&b->dcount, 2 * sizeof(int)));
     if (!b->rare.empty()) {  // largest first, each to the least loaded stream
         std::vector<size_t> order(b->rare.size());
         for (size_t k = 0; k < order.size(); k++) order[k] = k;
@@ -1763,10 +1685,10 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
             load[best] += b->rare_items[k];
         }
         for (int k = 0; k < cgh_batch::RSTREAMS - 1; k++) {
-            HK(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking));
-            HK(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming));
+            HIP_TRY(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming));
         }
-        HK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     }
     *out = b;
     return 0;
@@ -1821,7 +1743,7 @@ hipError_t batch_enqueue(cgh_batch* b, uint16_t* d_out, hipStream_t st, int kern
         RareJob jb = b->rare[k];
         jb.out = d_out + b->offs[b->rare_ds[k]];
         const hipStream_t sk = b->rare_lane[k] ? b->side[b->rare_lane[k] - 1] : st;
-        hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(b->rare_items[k])), dim3(RNT), 0, sk, jb, b->rare_items[k]);
+        if ((e = launch_rare(jb, b->rare_items[k], sk)) != hipSuccess) return e;
     }
     for (int k = 0; k < cgh_batch::RSTREAMS - 1 && e == hipSuccess; k++) {
         e = hipEventRecord(b->join[k], b->side[k]);
@@ -1845,31 +1767,31 @@ extern "C" int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int ke
         set_err(err, errlen, "output buffer must be 16-byte aligned");
         return 1;
     }
-    HK(hipSetDevice(b->device));
+    HIP_TRY(hipSetDevice(b->device));
     if (!use_graph || st == NULL) {
-        HK(batch_enqueue(b, d_out, st, kernel));
+        HIP_TRY(batch_enqueue(b, d_out, st, kernel));
         return 0;
     }
     if (!b->exec || b->graph_out != d_out || b->graph_stream != st || b->graph_kernel != kernel) {
         if (b->exec) {
-            HK(hipGraphExecDestroy(b->exec));
+            HIP_TRY(hipGraphExecDestroy(b->exec));
             b->exec = NULL;
         }
-        HK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         {
             const hipError_t e = batch_enqueue(b, d_out, st, kernel);
             const hipError_t e2 = hipStreamEndCapture(st, &graph);
-            HK(e);
-            HK(e2);
+            HIP_TRY(e);
+            HIP_TRY(e2);
         }
-        HK(hipGraphInstantiate(&b->exec, graph, NULL, NULL, 0));
-        HK(hipGraphDestroy(graph));
+        HIP_TRY(hipGraphInstantiate(&b->exec, graph, NULL, NULL, 0));
+        HIP_TRY(hipGraphDestroy(graph));
         graph = NULL;
         b->graph_out = d_out;
         b->graph_stream = st;
         b->graph_kernel = kernel;
     }
-    HK(hipGraphLaunch(b->exec, st));
+    HIP_TRY(hipGraphLaunch(b->exec, st));
     return 0;
 fail:
     if (graph) (void)hipGraphDestroy(graph);
@@ -2001,9 +1923,9 @@ extern "C" int cgh_db_create(int device, const uint64_t* calls, int n, int L, in
     cgh_codec_info(db->codec, &db->n, &db->L, &db->tau, &db->kw1, &db->kw2, &db->D, &nmem);
     if (cgh_dataset_upload(device, &enc, &db->ds, err, errlen)) goto fail;
     cgh_encoding_free(&enc);
-    HK(hipSetDevice(device));
+    HIP_TRY(hipSetDevice(device));
     db->m16 = db->n <= 65536;
-    HK(upload_members(cgh_codec_members(db->codec), nmem, db->m16, &db->dqmem));
+    HIP_TRY(upload_members(cgh_codec_members(db->codec), nmem, db->m16, &db->dqmem));
     *out = db;
     return 0;
 fail:
@@ -2122,9 +2044,11 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     else if (fp4) hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, sl.st, a);
     else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);
     if (ne) {
-        RareJob jb = {sl.ddist, q, 0, q, 1, sl.drs, sl.dre, db->dqmem, 1, db->n, db->m16, 0, 1, 0, 32};
+        RareJob jb = rare_job(sl.ddist, q, 0, q, sl.drs, sl.dre, db->dqmem, db->m16, 0, 32);
+        jb.rect = 1;
+        jb.nB = db->n;
         const long long items = rare_layout(jb, q);
-        hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(items)), dim3(RNT), 0, sl.st, jb, items);
+        if ((e = launch_rare(jb, items, sl.st))) return e;
     }
     const int tchunks = (db->n + TKC - 1) / TKC;
     hipLaunchKernelGGL(top_k_part, dim3((unsigned)((size_t)q * tchunks)), dim3(256), 0, sl.st, sl.ddist, db->n,
@@ -2267,8 +2191,8 @@ extern "C" int cgh_server_create(cgh_db* db, int slots, int max_batch, size_t hb
         delete sv;
         return 1;
     }
-    HK(hipSetDevice(db->device));
-    HK(hipMemGetInfo(&freeb, &totb));
+    HIP_TRY(hipSetDevice(db->device));
+    HIP_TRY(hipMemGetInfo(&freeb, &totb));
     {
         const char* env = getenv("CGH_FREE_BYTES");
         if (env && *env) freeb = std::min(freeb, (size_t)strtoull(env, NULL, 10));
@@ -2289,25 +2213,25 @@ extern "C" int cgh_server_create(cgh_db* db, int slots, int max_batch, size_t hb
     for (auto& sl : sv->slots) {
         const size_t qp = up((size_t)max_batch, 256);
         sl.re_cap = (size_t)max_batch * db->L;
-        HK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
-        HK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
-        HK(hipMalloc(&sl.dOr, qp * db->kw1 * 4));
-        HK(hipMalloc(&sl.dPr, qp * db->kw2 * 4));
-        HK(hipMalloc(&sl.dO, qp * db->ds->kw1 * 4));
-        HK(hipMalloc(&sl.dP, qp * db->ds->kw2 * 4));
-        HK(hipMalloc(&sl.drs, ((size_t)max_batch + 1) * 4));
-        HK(hipMalloc(&sl.dre, std::max<size_t>(sl.re_cap, 1) * 8));
-        HK(hipMalloc(&sl.ddist, up((size_t)max_batch * db->n, 8) * 2));
-        HK(hipMalloc(&sl.dtopd, (size_t)max_batch * TOPK * 2));
-        HK(hipMalloc(&sl.dtopi, (size_t)max_batch * TOPK * 4));
-        HK(hipMalloc(&sl.dpart, (size_t)max_batch * ((db->n + TKC - 1) / TKC) * TOPK * 8));
-        HK(hipHostMalloc(&sl.hO, qp * db->kw1 * 4, hipHostMallocDefault));
-        HK(hipHostMalloc(&sl.hP, qp * db->kw2 * 4, hipHostMallocDefault));
-        HK(hipHostMalloc(&sl.hrs, ((size_t)max_batch + 1) * 4, hipHostMallocDefault));
-        HK(hipHostMalloc(&sl.hre, std::max<size_t>(sl.re_cap, 1) * 8, hipHostMallocDefault));
-        HK(hipHostMalloc(&sl.hdist, (size_t)max_batch * db->n * 2, hipHostMallocDefault));
-        HK(hipHostMalloc(&sl.htopd, (size_t)max_batch * TOPK * 2, hipHostMallocDefault));
-        HK(hipHostMalloc(&sl.htopi, (size_t)max_batch * TOPK * 4, hipHostMallocDefault));
+        HIP_TRY(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+        HIP_TRY(hipMalloc(&sl.dOr, qp * db->kw1 * 4));
+        HIP_TRY(hipMalloc(&sl.dPr, qp * db->kw2 * 4));
+        HIP_TRY(hipMalloc(&sl.dO, qp * db->ds->kw1 * 4));
+        HIP_TRY(hipMalloc(&sl.dP, qp * db->ds->kw2 * 4));
+        HIP_TRY(hipMalloc(&sl.drs, ((size_t)max_batch + 1) * 4));
+        HIP_TRY(hipMalloc(&sl.dre, std::max<size_t>(sl.re_cap, 1) * 8));
+        HIP_TRY(hipMalloc(&sl.ddist, up((size_t)max_batch * db->n, 8) * 2));
+        HIP_TRY(hipMalloc(&sl.dtopd, (size_t)max_batch * TOPK * 2));
+        HIP_TRY(hipMalloc(&sl.dtopi, (size_t)max_batch * TOPK * 4));
+        HIP_TRY(hipMalloc(&sl.dpart, (size_t)max_batch * ((db->n + TKC - 1) / TKC) * TOPK * 8));
+        HIP_TRY(hipHostMalloc(&sl.hO, qp * db->kw1 * 4, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&sl.hP, qp * db->kw2 * 4, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&sl.hrs, ((size_t)max_batch + 1) * 4, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&sl.hre, std::max<size_t>(sl.re_cap, 1) * 8, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&sl.hdist, (size_t)max_batch * db->n * 2, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&sl.htopd, (size_t)max_batch * TOPK * 2, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&sl.htopi, (size_t)max_batch * TOPK * 4, hipHostMallocDefault));
     }
     sv->worker = std::thread(serve_loop, sv);
     *out = sv;
@@ -2373,70 +2297,260 @@ extern "C" int cgh_server_wait(cgh_server* sv, long ticket, uint16_t* dist, uint
     return rc;
 }
 
-// Stripes streamed to the host: the reference pipeline's shape (tch.cu:82-170,
-// two device buffers, two streams, the host consumes stripe s while stripe
-// s+1 is computed) driven by the device-resident dataset, so the rare pairs
-// never have to be expanded into a per-pair list.
-extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, char* err,
-                                  int errlen) {
-    uint16_t *dU[2] = {NULL, NULL}, *hU[2] = {NULL, NULL};
-    hipStream_t st[2] = {NULL, NULL};
-    hipEvent_t done[2] = {NULL, NULL}, t0 = NULL, t1 = NULL;
-    int rc = 1, prev = -1;
-    float ms = 0.f;
-    const int n = ds->n, tile = ds_tile(ds, CGH_KERNEL_AUTO);
-    const size_t stripe_elems = up((size_t)S * ds->n_pad, 8);
-    if (S < 8 || S % 8) {
-        set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
+
+// ---------------------------------------------------------------------------
+// Stripes delivered to a host callback (tch_run and cgh_dataset_stream)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// A ring of SLOTS stripe buffers between the device and a host consumer.
+// Stripe s is computed into device slot s % SLOTS on the compute stream, its
+// rows x w cells are copied to pinned host slot s % SLOTS on a separate copy
+// stream (so the copy of stripe s runs under the computation of stripe
+// s + 1), and the consumer takes the stripes in order on the calling thread.
+// A slot is filled again only after the consumer has returned from the
+// stripe it held, so the device works at most SLOTS - 1 stripes ahead and
+// every buffer the consumer sees stays untouched while it reads it.
+class StripeRing {
+public:
+    static constexpr int SLOTS = 3;
+
+    StripeRing() = default;
+    StripeRing(const StripeRing&) = delete;
+    StripeRing& operator=(const StripeRing&) = delete;
+    ~StripeRing() {
+        // in-flight copies and kernels still reference the buffers
+        if (compute_) (void)hipStreamSynchronize(compute_);
+        if (copy_) (void)hipStreamSynchronize(copy_);
+        for (int k = 0; k < SLOTS; k++) {
+            if (dev_[k]) (void)hipFree(dev_[k]);
+            if (host_[k]) (void)hipHostFree(host_[k]);
+            if (computed_[k]) (void)hipEventDestroy(computed_[k]);
+            if (landed_[k]) (void)hipEventDestroy(landed_[k]);
+        }
+        if (compute_) (void)hipStreamDestroy(compute_);
+        if (copy_) (void)hipStreamDestroy(copy_);
+    }
+
+    // Buffers of `cells` u16 per slot (a multiple of 8: the rare pass stages
+    // whole 16-byte words); the streams are created at the default priority.
+    hipError_t open(size_t cells) {
+        hipError_t e = hipStreamCreateWithFlags(&compute_, hipStreamNonBlocking);
+        if (!e) e = hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking);
+        for (int k = 0; k < SLOTS && !e; k++) {
+            e = hipMalloc(&dev_[k], cells * 2);
+            if (!e) e = hipHostMalloc(&host_[k], cells * 2, hipHostMallocDefault);
+            if (!e) e = hipEventCreateWithFlags(&computed_[k], hipEventDisableTiming);
+            if (!e) e = hipEventCreateWithFlags(&landed_[k], hipEventDisableTiming);
+        }
+        return e;
+    }
+
+    // produce(r0, rows, dev, stream) enqueues stripe r0's device work into
+    // dev (rows x (n - r0) u16 layout) and returns its status;
+    // consume(r0, rows, w, host) is the host side of that stripe.
+    template <class Produce, class Consume>
+    hipError_t run(int n, int S, Produce&& produce, Consume&& consume) {
+        const int stripes = (n + S - 1) / S;
+        auto fill = [&](int s) -> hipError_t {
+            const int k = s % SLOTS, r0 = s * S, rows = std::min(S, n - r0);
+            const size_t cells = (size_t)rows * (size_t)(n - r0);
+            hipError_t e = produce(r0, rows, dev_[k], compute_);
+            if (!e) e = hipEventRecord(computed_[k], compute_);
+            if (!e) e = hipStreamWaitEvent(copy_, computed_[k], 0);
+            if (!e) e = hipMemcpyAsync(host_[k], dev_[k], cells * 2, hipMemcpyDeviceToHost, copy_);
+            if (!e) e = hipEventRecord(landed_[k], copy_);
+            return e;
+        };
+        hipError_t e = hipSuccess;
+        for (int s = 0; s < std::min(stripes, SLOTS) && !e; s++) e = fill(s);
+        for (int s = 0; s < stripes && !e; s++) {
+            const int k = s % SLOTS, r0 = s * S;
+            e = hipEventSynchronize(landed_[k]);
+            if (e) break;
+            consume(r0, std::min(S, n - r0), (size_t)(n - r0), host_[k]);
+            if (s + SLOTS < stripes) e = fill(s + SLOTS);
+        }
+        return e;
+    }
+
+private:
+    uint16_t* dev_[SLOTS] = {};
+    uint16_t* host_[SLOTS] = {};
+    hipEvent_t computed_[SLOTS] = {};
+    hipEvent_t landed_[SLOTS] = {};
+    hipStream_t compute_ = nullptr;
+    hipStream_t copy_ = nullptr;
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+
+// f(lo, hi) over [0, count) split into contiguous ranges on the host's
+// hardware threads.
+template <class F>
+void host_parallel(size_t count, F&& f) {
+    const size_t threads = std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), 64));
+    const size_t per = (count + threads - 1) / threads;
+    if (threads == 1 || count < 2 * threads) {
+        f((size_t)0, count);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t lo = 0; lo < count; lo += per) pool.emplace_back([&f, lo, per, count] { f(lo, std::min(count, lo + per)); });
+    for (auto& t : pool) t.join();
+}
+
+// Device copy of a caller's rare CSR (start[n + 1], js[n_rare]: partners
+// j > i of row i, in any order), read by the row-local rare pass as one
+// member range per row that has partners: the partner array is the member
+// array and row i's only range is [start[i], start[i + 1]).
+struct CsrRanges {
+    uint32_t* members = nullptr;  // js
+    uint32_t* rstart = nullptr;   // n + 1 offsets into rentry
+    uint64_t* rentry = nullptr;   // (end << 32) | begin
+    size_t entries = 0;
+
+    ~CsrRanges() {
+        if (members) (void)hipFree(members);
+        if (rstart) (void)hipFree(rstart);
+        if (rentry) (void)hipFree(rentry);
+    }
+    static size_t device_bytes(int n, size_t n_rare) {
+        return std::max<size_t>(n_rare, 1) * 4 + ((size_t)n + 1) * 4 + (size_t)n * 8;
+    }
+    hipError_t upload(int n, const uint32_t* start, const uint32_t* js, size_t n_rare) {
+        std::vector<uint32_t> rs((size_t)n + 1);
+        std::vector<uint64_t> re;
+        re.reserve((size_t)n);
+        for (int i = 0; i < n; i++) {
+            rs[(size_t)i] = (uint32_t)re.size();
+            if (start[i + 1] > start[i]) re.push_back(((uint64_t)start[i + 1] << 32) | start[i]);
+        }
+        rs[(size_t)n] = (uint32_t)re.size();
+        entries = re.size();
+        hipError_t e = hipMalloc(&members, std::max<size_t>(n_rare, 1) * 4);
+        if (!e) e = hipMalloc(&rstart, rs.size() * 4);
+        if (!e) e = hipMalloc(&rentry, std::max<size_t>(entries, 1) * 8);
+        if (!e && n_rare) e = hipMemcpy(members, js, n_rare * 4, hipMemcpyHostToDevice);
+        if (!e) e = hipMemcpy(rstart, rs.data(), rs.size() * 4, hipMemcpyHostToDevice);
+        if (!e && entries) e = hipMemcpy(rentry, re.data(), entries * 8, hipMemcpyHostToDevice);
+        return e;
+    }
+};
+
+// Device operand planes of a caller's row-major bit matrices, freed with
+// the object.
+struct OwnedPlanes {
+    uint32_t* O = nullptr;
+    uint32_t* P = nullptr;
+    ~OwnedPlanes() {
+        if (O) (void)hipFree(O);
+        if (P) (void)hipFree(P);
+    }
+};
+
+bool stripe_height_ok(int S, char* err, int errlen) {
+    if (S >= 8 && S % 8 == 0) return true;
+    set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
+    return false;
+}
+
+}  // namespace
+
+// The reference boundary as an adapter onto the library's own parts: the
+// caller's bit matrices become device planes, its CSR becomes member ranges
+// of the row-local rare pass (or, when it does not fit in device memory,
+// is applied on the host to each stripe before the callback), and the
+// stripes flow through the StripeRing.
+extern "C" int tch_run(int device, int n, size_t n_pad, const uint8_t* O, size_t K1, const uint8_t* P,
+                       size_t K2, const uint32_t* start, const uint32_t* js, size_t n_rare, int S, tch_cb cb,
+                       void* ctx, double* gpu_secs, int* rare_on_gpu, char* err, int errlen) {
+    if (n < 1 || K1 == 0 || K2 == 0 || K1 % 1024 || K2 % 1024 || n_pad % 256 || n_pad < (size_t)n ||
+        n_pad > (size_t)INT32_MAX) {
+        set_err(err, errlen, "tch_run: bad shape (n=%d n_pad=%zu K1=%zu K2=%zu)", n, n_pad, K1, K2);
         return 1;
     }
-    HK(hipSetDevice(ds->device));
-    for (int b = 0; b < 2; b++) {
-        HK(hipMalloc(&dU[b], stripe_elems * 2));
-        HK(hipHostMalloc(&hU[b], stripe_elems * 2, hipHostMallocDefault));
-        HK(hipStreamCreateWithFlags(&st[b], hipStreamNonBlocking));
-        HK(hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
-    }
-    HK(hipEventCreate(&t0));
-    HK(hipEventCreate(&t1));
-    HK(hipEventRecord(t0, st[0]));
-    for (int r0 = 0;; r0 += S) {
-        if (r0 < n) {
-            const int rows = std::min(S, n - r0), b = (r0 / S) & 1;
-            const size_t w = (size_t)(n - r0);
-            HK(launch_tiles(ds_kernel(ds, CGH_KERNEL_AUTO), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2,
-                            (int)ds->n_pad, n, r0, S, 1, dU[b], st[b], ds->drstart, ds->cap));
-            if (ds->n_rare && n - r0 > 1) {
-                RareJob jb = {dU[b], n, r0, S, 1, ds->drstart, ds->drentry, ds->dgmem, 0, 0, ds->m16, ds->cap, 1,
-                              0, lanes_per_range(ds->n_rare, ds->n_rentries)};
-                const long long items = rare_layout(jb, rows);
-                hipLaunchKernelGGL(sub_rare_rows, dim3(rare_grid(items)), dim3(RNT), 0, st[b], jb, items);
-                HK(hipGetLastError());
+    if (!stripe_height_ok(S, err, errlen)) return 1;
+    const auto t_start = std::chrono::steady_clock::now();
+    const size_t slot_cells = up((size_t)S * (size_t)n, 8);
+    const int kw1 = kc_words(K1 / 32), kw2 = kc_words(K2 / 32);
+    const int kernel = resolve_kernel(CGH_KERNEL_AUTO), tile = pick_tile(n);
+    OwnedPlanes planes;
+    CsrRanges csr;
+    StripeRing ring;
+    size_t free_b = 0, total_b = 0;
+    bool on_device = false;
+
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(upload_planes(O, (int)n_pad, (int)(K1 / 32), kw1, &planes.O));
+    HIP_TRY(upload_planes(P, (int)n_pad, (int)(K2 / 32), kw2, &planes.P));
+    HIP_TRY(ring.open(slot_cells));
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (const char* env = getenv("CGH_FREE_BYTES"); env && *env) free_b = std::min(free_b, (size_t)strtoull(env, NULL, 10));
+    // rare pairs on the device when their ranges fit beside a 64 MiB margin
+    on_device = n_rare == 0 || CsrRanges::device_bytes(n, n_rare) + (64u << 20) <= free_b;
+    if (n_rare && on_device) HIP_TRY(csr.upload(n, start, js, n_rare));
+    if (rare_on_gpu) *rare_on_gpu = on_device;
+    {
+        const int rsg = lanes_per_range(n_rare, csr.entries);
+        auto produce = [&](int r0, int rows, uint16_t* dst, hipStream_t st) -> hipError_t {
+            hipError_t e = launch_tiles(kernel, tile, planes.O, kw1, planes.P, kw2, (int)n_pad, n, r0, S, 1, dst, st);
+            if (!e && n_rare && on_device && n - r0 > 1) {
+                RareJob jb = rare_job(dst, n, r0, S, csr.rstart, csr.rentry, csr.members, 0, 0, rsg);
+                jb.sorted = 0;  // a CSR row lists its partners in any order
+                e = launch_rare(jb, rare_layout(jb, rows), st);
             }
-            HK(hipMemcpyAsync(hU[b], dU[b], (size_t)rows * w * 2, hipMemcpyDeviceToHost, st[b]));
-            HK(hipEventRecord(done[b], st[b]));
-        }
-        if (prev >= 0) {  // consume the previous stripe while this one runs
-            const int rows = std::min(S, n - prev), b = (prev / S) & 1;
-            HK(hipEventSynchronize(done[b]));
-            cb(ctx, prev, rows, (size_t)(n - prev), hU[b]);
-        }
-        if (r0 >= n) break;
-        prev = r0;
+            return e;
+        };
+        auto consume = [&](int r0, int rows, size_t w, uint16_t* cells) {
+            if (n_rare && !on_device)  // partner j of row r0 + r is column j - r0 of stripe row r
+                host_parallel((size_t)rows, [&](size_t lo, size_t hi) {
+                    for (size_t r = lo; r < hi; r++) {
+                        uint16_t* row = cells + r * w - r0;
+                        const size_t i = (size_t)r0 + r;
+                        for (const uint32_t* p = js + start[i]; p != js + start[i + 1]; ++p) row[*p]--;
+                    }
+                });
+            cb(ctx, r0, rows, w, cells);
+        };
+        HIP_TRY(ring.run(n, S, produce, consume));
     }
-    HK(hipEventRecord(t1, st[0]));
-    HK(hipEventSynchronize(t1));
-    HK(hipEventElapsedTime(&ms, t0, t1));
-    if (gpu_secs) *gpu_secs = ms / 1e3;
-    rc = 0;
+    if (gpu_secs) *gpu_secs = seconds_since(t_start);
+    return 0;
 fail:
-    for (int b = 0; b < 2; b++) {
-        if (dU[b]) (void)hipFree(dU[b]);
-        if (hU[b]) (void)hipHostFree(hU[b]);
-        if (st[b]) (void)hipStreamDestroy(st[b]);
-        if (done[b]) (void)hipEventDestroy(done[b]);
+    return 1;
+}
+
+// The CLI's path: stripes of a device-resident dataset through the
+// StripeRing, each stripe one product launch (with the cap of rows that have
+// no rare partners) and one row-local rare launch over the dataset's groups.
+extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, char* err,
+                                  int errlen) {
+    if (!stripe_height_ok(S, err, errlen)) return 1;
+    const auto t_start = std::chrono::steady_clock::now();
+    const int n = ds->n, kernel = ds_kernel(ds, CGH_KERNEL_AUTO), tile = ds_tile(ds, CGH_KERNEL_AUTO);
+    const int rsg = lanes_per_range(ds->n_rare, ds->n_rentries);
+    StripeRing ring;
+    HIP_TRY(hipSetDevice(ds->device));
+    HIP_TRY(ring.open(up((size_t)S * (size_t)n, 8)));
+    {
+        auto produce = [&](int r0, int rows, uint16_t* dst, hipStream_t st) -> hipError_t {
+            hipError_t e = launch_tiles(kernel, tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n,
+                                        r0, S, 1, dst, st, ds->drstart, ds->cap);
+            if (!e && ds->n_rare && n - r0 > 1) {
+                RareJob jb = rare_job(dst, n, r0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap, rsg);
+                e = launch_rare(jb, rare_layout(jb, rows), st);
+            }
+            return e;
+        };
+        auto consume = [&](int r0, int rows, size_t w, const uint16_t* cells) { cb(ctx, r0, rows, w, cells); };
+        HIP_TRY(ring.run(n, S, produce, consume));
     }
-    if (t0) (void)hipEventDestroy(t0);
-    if (t1) (void)hipEventDestroy(t1);
-    return rc;
+    if (gpu_secs) *gpu_secs = seconds_since(t_start);
+    return 0;
+fail:
+    return 1;
 }

@@ -28,6 +28,17 @@ struct DatasetParts {
 
 cgh_dataset* dataset_adopt(const DatasetParts& p);
 
+// True when e is an error; then "HIP error '<text>' in <call>" goes to err
+// (may be NULL, truncated to errlen bytes).
+bool hip_failed(hipError_t e, const char* call, char* err, int errlen);
+
+// Inside an entry point that has `err`, `errlen` and a `fail:` cleanup
+// label: report the first failing HIP call and leave through the label.
+#define HIP_TRY(x)                                                     \
+    do {                                                               \
+        if (cghi::hip_failed((x), #x, err, errlen)) goto fail;         \
+    } while (0)
+
 // Device words per row for host words kw (rounded up to the product slab).
 int device_words(size_t kw);
 

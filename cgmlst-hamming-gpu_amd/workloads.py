@@ -9,6 +9,9 @@ Shapes follow the benchmark configurations of this project:
   founders     n x L, 50 founder alleles per locus, each profile copies a
                founder per locus, 1% mutated to a fresh id, 1% missing
 
+  hash_ids_fast
+               per-locus multiplicative relabelling of ids to 32 / 64 bits
+
 Each returns u64 allele ids, n x L row-major, 0 = missing.
 """
 from __future__ import annotations
@@ -122,3 +125,23 @@ def serving_sets(n_db, L, n_queries, seed, fresh=0.03, a_min=3, a_max=600):
     q[new] = (10**12 + np.arange(int(new.sum()))).astype(np.uint64)
     calls = hashed_ids(calls, bits=64, seed=seed)
     return calls[:n_db], calls[n_db:]
+
+
+def hash_ids_fast(calls, bits, seed):
+    """Relabel every id of locus k as id * M_k modulo 2^bits with a random odd
+    M_k: a bijection on each locus's ids (so equality is kept), 0 stays 0
+    (missing), and the rank order of the new ids is scrambled."""
+    rng = np.random.default_rng(seed)
+    L = calls.shape[1]
+    m = (rng.integers(0, 2**62, size=L, dtype=np.uint64) * np.uint64(2) + np.uint64(1))
+    out = calls.astype(np.uint64) * m[None, :]  # wraps modulo 2^64
+    if bits == 32:
+        out &= np.uint64(0xFFFFFFFF)
+    return out
+
+
+def batch_dataset(n, L, seed, d):
+    """Dataset d of the 64-set batching workload (shape from batch_shapes):
+    Zipf(1.2) long tail, 2% missing, ids relabelled to 32 bits in even
+    datasets and 64 bits in odd ones."""
+    return hash_ids_fast(zipf_iid(n, L, seed, s=1.2), 32 if d % 2 == 0 else 64, seed + 17)

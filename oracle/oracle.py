@@ -303,11 +303,23 @@ def rust_lines(text):
             for i, p in enumerate(parts)]
 
 
+# Rust's str::trim removes char::is_whitespace from both ends: the Unicode
+# White_Space property (Rust core docs), which Python's str.strip does not
+# follow (it also strips U+001C..U+001F).
+RUST_WHITE_SPACE = ("\t\n\x0b\x0c\r \x85\xa0\u1680" + "".join(chr(c) for c in range(0x2000, 0x200B)) +
+                    "\u2028\u2029\u202f\u205f\u3000")
+
+
+def rust_trim(s):
+    """str::trim (main.rs:99): strip White_Space characters from both ends."""
+    return s.strip(RUST_WHITE_SPACE)
+
+
 def parse_allele(cell, skip_repl=False, missing="-"):
     """main.rs:95-113: trim, strip one leading INF- (unless skip_repl), the
     missing value or empty -> 0, all ASCII digits -> u64 (overflow raises),
     anything else -> 0."""
-    s = cell.strip()
+    s = rust_trim(cell)
     if not skip_repl and s.startswith("INF-"):
         s = s[4:]
     if s == missing or s == "":

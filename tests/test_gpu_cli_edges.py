@@ -48,7 +48,10 @@ def mixed_text(rng, n, L, crlf=False):
     """Every kind of cell main.rs:95-113 distinguishes, ragged rows and empty
     lines."""
     kinds = ["-", "", "LNF", "PLOT3", "+5", "5.0", "0", "007", "INF-INF-5", "INF- 5", " 7 ",
-             " 12 ", "\u00a012\u2003", "INF-12", "INF-", "12", "3", "3", "3", "4", "18446744073709551615"]
+             " 12 ", "\u00a012\u2003", "INF-12", "INF-", "12", "3", "3", "3", "4", "18446744073709551615",
+             # Rust's trim strips White_Space only: U+001C..U+001F and U+200B stay (the cell is then not a
+             # number), U+0085, U+000B, U+000C and U+3000 go
+             "\x1c5", "5\x1f", "\x1e12", "\u200b3", "\x853\x85", "\x0b4\x0c", "\u300012"]
     lines = ["FILE\t" + "\t".join(f"L{j}" for j in range(L))]
     for i in range(n):
         w = L + int(rng.integers(-3, 4))

@@ -156,19 +156,44 @@ def test_autotune_then_exact(cgh, gpu, workloads, shape):
 
 
 @pytest.mark.parametrize("force_host", [False, True])
-def test_tch_run(cgh, gpu, monkeypatch, force_host):
-    """The reference-compatible tch_run entry point, rare pairs on the GPU
-    and (free memory forced to 1 MiB) on the host."""
-    A, _ = oracle.check_make(700, 410, 2, True)
+@pytest.mark.parametrize("shape", [(700, 410, 2, 64, 64), (1500, 64, 3, 2, 512), (1500, 64, 3, 4096, 8)])
+def test_tch_run(cgh, gpu, monkeypatch, force_host, shape):
+    """The reference-compatible tch_run entry point on check.py's datasets,
+    two shapes x stripe heights (S = 8 makes a stripe narrower than a
+    tile), rare pairs on the GPU and (free memory forced to 1 MiB) on the
+    host, each row's partners in a shuffled order."""
+    n, L, seed, tau, S = shape
+    A, _ = oracle.check_make(n, L, seed, True)
     X = oracle.ranks_checkpy_style(A)
     if force_host:
         monkeypatch.setenv("CGH_FREE_BYTES", str(1 << 20))
-    enc = cgh.Encoded(X, 64, 1024)
-    print(f"tch_run force_host={force_host}", flush=True)
-    H, secs, rare_gpu, seen = cgh.tch_run(enc, S=64)
-    assert rare_gpu == (not force_host)
-    assert [s[0] for s in seen] == list(range(0, 700, 64))
+    enc = cgh.Encoded(X, tau, 1024)
+    print(f"tch_run n={n} tau={tau} S={S} force_host={force_host}", flush=True)
+    H, secs, rare_gpu, seen = cgh.tch_run(enc, S=S, shuffle_seed=seed)
+    assert rare_gpu == (not force_host or enc.enc.n_rare == 0)
+    assert [s[0] for s in seen] == list(range(0, n, S))
+    assert [s[2] for s in seen] == [n - r0 for r0 in range(0, n, S)]
+    assert secs > 0
     assert np.array_equal(H, oracle.check_ref(A).astype(np.uint16))
+
+
+def test_tch_run_rows_wider_than_one_chunk(cgh, gpu):
+    """tch_run with n = 70,000: rows with more than 65,536 cells right of
+    the diagonal are staged in two column chunks, and a CSR row's partners
+    (here shuffled) are walked whole in each chunk with the members outside
+    it masked; rows on both sides of the split against the oracle."""
+    rng = np.random.default_rng(23)
+    n, L = 70000, 4
+    X = rng.integers(1, 2000, size=(n, L)).astype(np.uint16)  # ~35 copies per allele: all rare at tau 64
+    X[rng.random((n, L)) < 0.03] = 0
+    enc = cgh.Encoded(X, 64, 1024)
+    assert enc.enc.n_rare > 10**6
+    rows = [0, 1, 4462, 4463, 4464, 30000, 69998]
+    print("tch_run n=70000 S=4096", flush=True)
+    got, _, rare_gpu, _ = cgh.tch_run(enc, S=4096, keep_rows=rows, shuffle_seed=5)
+    assert rare_gpu
+    for i in rows:
+        assert np.array_equal(got[i], oracle.brute_rows(X, i, i + 1)[0, i + 1:]), i
 
 
 @pytest.mark.parametrize("S,tau", [(64, 8), (520, 64), (4096, 1024), (8, 2)])
@@ -324,3 +349,44 @@ def test_rare_pass_many_chunks(cgh, gpu):
         got = out[off + (i + 1 - r0): off + w].cpu().numpy().view(np.uint16)
         ref = oracle.brute_rows(X, i, i + 1)[0, i + 1:]
         assert np.array_equal(got, ref), i
+
+
+@pytest.mark.parametrize("cap", [4464, 65535, 65536, 70000])
+def test_cap_at_and_beyond_u16(cgh, gpu, cap):
+    """-X at and past the u16 range on rows that the rare pass finishes (tau
+    4096: every allele of these 300 rows is rare): a cap from 65,535 up
+    changes nothing, since every distance is <= L, whatever the cap is modulo
+    2^16 (70,000 = 65,536 + 4,464).  Dataset path, batch path and the CLI."""
+    rng = np.random.default_rng(21)
+    n, L, S = 300, 5000, 128
+    X = rng.integers(1, 21, size=(n, L)).astype(np.uint16)   # 20 alleles: H ~ 0.95 L > 4,464
+    X[rng.random((n, L)) < 0.02] = 0
+    ref = oracle.brute_full(X)
+    assert ref.max() > 4464
+    want = np.minimum(ref, cap) if cap < 65535 else ref
+    print(f"cap {cap}: dataset", flush=True)
+    H, _, enc = gpu_upper(cgh, gpu, X, 4096, S, cap=cap)
+    assert enc.enc.n_rare > 0
+    assert np.array_equal(H, want)
+    ds = cgh.Dataset(cgh.Encoded(X, 4096, 256))
+    ds.set_cap(cap)
+    b = cgh.Batch([ds], S)
+    out = gpu.zeros(b.out_elems(), dtype=gpu.int16, device="cuda")
+    print(f"cap {cap}: batch", flush=True)
+    b.run(out.data_ptr(), gpu.cuda.current_stream().cuda_stream, 0, graph=False)
+    gpu.cuda.synchronize()
+    Hb = cgh.upper_to_square(out.cpu().numpy().view(np.uint16)[: cgh.upper_elems(n, S)], n, S)
+    assert np.array_equal(Hb, want)
+    b.close()
+    ds.close()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "in.tsv")
+        text = "FILE\t" + "\t".join(f"L{k}" for k in range(L)) + "\n"
+        text += "".join(f"S{i}\t" + "\t".join(str(v) if v else "-" for v in r) + "\n" for i, r in enumerate(X))
+        with open(path, "w") as f:
+            f.write(text)
+        print(f"cap {cap}: cli", flush=True)
+        out = subprocess.run([cgh.CLI_PATH, "-s", "-i", path, "--tau", "4096", "-X", str(cap), "--stripe", "64"],
+                             capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        assert out.stdout == oracle.render_matrix([f"S{i}" for i in range(n)], ref, cap=cap)

@@ -123,7 +123,8 @@ def test_cli_device_and_separator_errors(cgh):
         pytest.skip("a GPU is visible: the device-missing path cannot be reached")
     cli = cgh.CLI_PATH
     r = subprocess.run([cli, "-i", "/nonexistent"], capture_output=True, text=True)
-    assert r.returncode == 1 and r.stderr.startswith("error: GPU device 0 not available")
+    # main.rs:283's text, kept verbatim by the drop-in
+    assert r.returncode == 1 and r.stderr == "error: CUDA device 0 not available\n"
     r = subprocess.run([cli, "-i", "/nonexistent", "-d", "ab"], capture_output=True, text=True)
     assert r.returncode == 1 and "input separator must be a single character" in r.stderr
 

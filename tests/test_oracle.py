@@ -213,3 +213,16 @@ def test_render_matrix_known_text():
     assert oracle.render_matrix(ids, H, mode="upper-tri") == \
         "cgmlst-hamming-gpu\ta\tb\tc\na\t0\t3\t9\nb\t0\t0\t1\nc\t0\t0\t0\n"
     assert oracle.render_matrix([], np.zeros((0, 0))) == "cgmlst-hamming-gpu\n"
+
+
+def test_rust_trim_is_white_space():
+    """main.rs:99 trims with str::trim, i.e. char::is_whitespace, the Unicode
+    White_Space property.  Python's str.isspace covers exactly White_Space
+    plus the four information separators U+001C..U+001F (bidi class B/S), so
+    the oracle's set must be isspace minus those four, over every code point."""
+    py_space = {c for c in range(0x110000) if chr(c).isspace()}
+    assert {ord(c) for c in oracle.RUST_WHITE_SPACE} == py_space - {0x1C, 0x1D, 0x1E, 0x1F}
+    assert oracle.parse_allele("\x1c5") == 0 and oracle.parse_allele("5\x1f") == 0
+    assert oracle.parse_allele("\x855\u3000") == 5 and oracle.parse_allele("\u200b5") == 0
+    # a planted mistake (Python's own strip) fails the first check
+    assert "\x1c5".strip() == "5"
