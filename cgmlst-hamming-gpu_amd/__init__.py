@@ -34,7 +34,7 @@ EXPORTS = (
     "cgh_codec_create", "cgh_codec_info", "cgh_codec_members", "cgh_codec_encode", "cgh_codec_free",
     "cgh_db_create", "cgh_db_info", "cgh_db_free",
     "cgh_server_create", "cgh_server_slots", "cgh_server_submit", "cgh_server_wait", "cgh_server_free",
-    "cgh_measure_peaks", "cgh_dataset_tile",
+    "cgh_measure_peaks", "cgh_measure_memory", "cgh_dataset_tile",
 )
 
 PHASE_PRODUCT, PHASE_RARE = 1, 2
@@ -117,6 +117,7 @@ def load():
         "cgh_server_wait": (I, [P, C.c_long, P, P, P, C.POINTER(C.c_double), C.c_char_p, I]),
         "cgh_server_free": (None, [P]),
         "cgh_dataset_tile": (I, [P, I]),
+        "cgh_measure_memory": (I, [I] + [C.POINTER(C.c_double)] * 3 + [C.c_char_p, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -155,6 +156,15 @@ def measure_peaks(device: int = 0) -> dict:
     _check(load().cgh_measure_peaks(device, *[C.byref(x) for x in v], err, 512), err)
     return {"int8_mfma_tops": v[0].value, "fp4_mfma_tflops": v[1].value, "valu_popcount_tops": v[2].value,
             "copy_gbs": v[3].value}
+
+
+def measure_memory(device: int = 0) -> dict:
+    """Streaming copy, read-only stream and cache-resident read rates in GB/s
+    (cgh_measure_memory)."""
+    v = [C.c_double() for _ in range(3)]
+    err = _err()
+    _check(load().cgh_measure_memory(device, *[C.byref(x) for x in v], err, 512), err)
+    return {"copy_gbs": v[0].value, "read_gbs": v[1].value, "cache_read_gbs": v[2].value}
 
 
 def dense_ranks(calls: np.ndarray) -> np.ndarray:

@@ -1561,6 +1561,12 @@ extern "C" int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* 
 // Many independent datasets in one persistent launch
 // ---------------------------------------------------------------------------
 
+namespace {
+// Tile edge of a dataset inside a batch: 256 from 4,096 rows (one persistent
+// queue keeps the chip busy with fewer, larger tiles), else 128.
+int batch_tile(int n) { return n >= 4096 ? 256 : 128; }
+}  // namespace
+
 struct cgh_batch {
     int device, count, S;
     int cus = 0;                        // compute units: persistent grid size
@@ -1630,7 +1636,7 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         // fill the chip, else 128
         // (a dataset's own tuned tile is not used: inside one persistent
         // launch 256-tiles pay off from smaller n than in a launch of its own)
-        const int tl = tile ? tile : (ds->n >= 4096 ? 256 : 128);
+        const int tl = tile ? tile : batch_tile(ds->n);
         b->tile.push_back(tl);
         b->offs.push_back(b->total);
         b->total += cgh_upper_elems(ds->n, S);

@@ -12,7 +12,10 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <algorithm>
+
 #include "cgmlst_hamming.h"
+#include "internal.h"
 
 namespace {
 
@@ -78,28 +81,43 @@ __global__ void __launch_bounds__(256) peak_popc(int iters, int* sink) {
     if (s == 0xffffffffu) sink[0] = (int)s;
 }
 
-__global__ void __launch_bounds__(256) stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
-        dst[k] = src[k];
+// Streaming copy: U independent 16-byte loads in flight per lane, each
+// word read and written once, non-temporal (no reuse to keep in the caches).
+template <int U>
+__global__ void __launch_bounds__(256) stream_copy(const v4i* __restrict__ src, v4i* __restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    for (; k + (U - 1) * stride < n; k += U * stride) {
+        v4i v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(&src[k + u * stride]);
+#pragma unroll
+        for (int u = 0; u < U; u++) __builtin_nontemporal_store(v[u], &dst[k + u * stride]);
+    }
+    for (; k < n; k += stride) dst[k] = src[k];
 }
 
-void set_err(char* err, int errlen, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
-void set_err(char* err, int errlen, const char* fmt, ...) {
-    if (!err || errlen <= 0) return;
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(err, (size_t)errlen, fmt, ap);
-    va_end(ap);
+// Read-only stream: `passes` sweeps over n 16-byte words, U loads in flight
+// per lane; a word-sum that is never the sentinel keeps the loads alive.
+template <int U>
+__global__ void __launch_bounds__(256) stream_read(const v4i* __restrict__ src, size_t n, int passes, int* sink) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    int acc = 0;
+    for (int p = 0; p < passes; p++) {
+        size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+        for (; k + (U - 1) * stride < n; k += U * stride) {
+            v4i v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = src[k + u * stride];
+#pragma unroll
+            for (int u = 0; u < U; u++) acc ^= v[u][0] + v[u][1] + v[u][2] + v[u][3];
+        }
+        asm volatile("" : : : "memory");  // every pass loads again (no hoisting across passes)
+    }
+    if (acc == 0x7fffffff) sink[0] = acc;
 }
 
-#define PK(x)                                                                  \
-    do {                                                                       \
-        hipError_t e_ = (x);                                                   \
-        if (e_ != hipSuccess) {                                                \
-            set_err(err, errlen, "%s: %s", #x, hipGetErrorString(e_));         \
-            goto fail;                                                         \
-        }                                                                      \
-    } while (0)
+
 
 }  // namespace
 
@@ -113,55 +131,124 @@ extern "C" int cgh_measure_peaks(int device, double* i8_tops, double* fp4_tflops
     int rc = 1;
     const int iters = 4000;
     const size_t bytes = (size_t)2 << 30;
-    PK(hipSetDevice(device));
-    PK(hipGetDeviceProperties(&prop, device));
-    PK(hipMalloc(&sink, 4));
-    PK(hipEventCreate(&t0));
-    PK(hipEventCreate(&t1));
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    HIP_TRY(hipMalloc(&sink, 4));
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
     {
         // 4 wavefronts per workgroup (one per SIMD), 2 workgroups per CU
         const dim3 grid((unsigned)prop.multiProcessorCount * 2), block(256);
         const double waves = (double)grid.x * 4.0;
         hipLaunchKernelGGL(peak_i8, grid, block, 0, 0, 16, sink);
-        PK(hipEventRecord(t0));
+        HIP_TRY(hipEventRecord(t0));
         hipLaunchKernelGGL(peak_i8, grid, block, 0, 0, iters, sink);
-        PK(hipEventRecord(t1));
-        PK(hipEventSynchronize(t1));
-        PK(hipEventElapsedTime(&ms, t0, t1));
+        HIP_TRY(hipEventRecord(t1));
+        HIP_TRY(hipEventSynchronize(t1));
+        HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
         if (i8_tops) *i8_tops = waves * iters * CHAINS * 2.0 * 32 * 32 * 32 / (ms * 1e-3) / 1e12;
         hipLaunchKernelGGL(peak_fp4, grid, block, 0, 0, 16, sink);
-        PK(hipEventRecord(t0));
+        HIP_TRY(hipEventRecord(t0));
         hipLaunchKernelGGL(peak_fp4, grid, block, 0, 0, iters, sink);
-        PK(hipEventRecord(t1));
-        PK(hipEventSynchronize(t1));
-        PK(hipEventElapsedTime(&ms, t0, t1));
+        HIP_TRY(hipEventRecord(t1));
+        HIP_TRY(hipEventSynchronize(t1));
+        HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
         if (fp4_tflops) *fp4_tflops = waves * iters * CHAINS * 2.0 * 32 * 32 * 64 / (ms * 1e-3) / 1e12;
         // popcount: 8 waves per CU-SIMD pair of launches keeps the VALU fed
         const dim3 pgrid((unsigned)prop.multiProcessorCount * 8);
         hipLaunchKernelGGL(peak_popc, pgrid, block, 0, 0, 16, sink);
-        PK(hipEventRecord(t0));
+        HIP_TRY(hipEventRecord(t0));
         hipLaunchKernelGGL(peak_popc, pgrid, block, 0, 0, iters * 4, sink);
-        PK(hipEventRecord(t1));
-        PK(hipEventSynchronize(t1));
-        PK(hipEventElapsedTime(&ms, t0, t1));
+        HIP_TRY(hipEventRecord(t1));
+        HIP_TRY(hipEventSynchronize(t1));
+        HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
         // one AND + one BCNT per 32 bit-pairs; counted as 2 ops per bit pair
         if (popc_tops)
             *popc_tops = (double)pgrid.x * 256.0 * iters * 4.0 * CHAINS * 32.0 * 2.0 / (ms * 1e-3) / 1e12;
     }
-    PK(hipMalloc(&src, bytes));
-    PK(hipMalloc(&dst, bytes));
-    PK(hipMemset(src, 1, bytes));
+    HIP_TRY(hipMalloc(&src, bytes));
+    HIP_TRY(hipMalloc(&dst, bytes));
+    HIP_TRY(hipMemset(src, 1, bytes));
     {
-        const dim3 grid((unsigned)prop.multiProcessorCount * 16), block(256);
-        hipLaunchKernelGGL(stream_copy, grid, block, 0, 0, src, dst, bytes / 16);
-        PK(hipEventRecord(t0));
-        for (int r = 0; r < 5; r++) hipLaunchKernelGGL(stream_copy, grid, block, 0, 0, src, dst, bytes / 16);
-        PK(hipEventRecord(t1));
-        PK(hipEventSynchronize(t1));
-        PK(hipEventElapsedTime(&ms, t0, t1));
-        if (hbm_gbs) *hbm_gbs = 5.0 * 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
+        // the best of a few grid sizes of the 4-loads-in-flight copy
+        double best = 0;
+        for (unsigned per_cu : {4u, 8u, 16u}) {
+            const dim3 grid((unsigned)prop.multiProcessorCount * per_cu), block(256);
+            const v4i* s4 = reinterpret_cast<const v4i*>(src);
+            v4i* d4 = reinterpret_cast<v4i*>(dst);
+            hipLaunchKernelGGL(stream_copy<4>, grid, block, 0, 0, s4, d4, bytes / 16);
+            HIP_TRY(hipEventRecord(t0));
+            for (int r = 0; r < 5; r++) hipLaunchKernelGGL(stream_copy<4>, grid, block, 0, 0, s4, d4, bytes / 16);
+            HIP_TRY(hipEventRecord(t1));
+            HIP_TRY(hipEventSynchronize(t1));
+            HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+            best = std::max(best, 5.0 * 2.0 * (double)bytes / (ms * 1e-3) / 1e9);
+        }
+        if (hbm_gbs) *hbm_gbs = best;
     }
-    PK(hipGetLastError());
+    HIP_TRY(hipGetLastError());
+    rc = 0;
+fail:
+    if (sink) (void)hipFree(sink);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    return rc;
+}
+
+extern "C" int cgh_measure_memory(int device, double* copy_gbs, double* read_gbs, double* cache_read_gbs, char* err,
+                                  int errlen) {
+    int* sink = NULL;
+    v4i *src = NULL, *dst = NULL;
+    hipEvent_t t0 = NULL, t1 = NULL;
+    hipDeviceProp_t prop;
+    float ms = 0.f;
+    int rc = 1;
+    const size_t bytes = (size_t)4 << 30, cached = (size_t)64 << 20;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    HIP_TRY(hipMalloc(&sink, 4));
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipMalloc(&src, bytes));
+    HIP_TRY(hipMalloc(&dst, bytes));
+    HIP_TRY(hipMemset(src, 3, bytes));
+    {
+        const dim3 block(256);
+        double best_copy = 0, best_read = 0, best_cache = 0;
+        for (unsigned per_cu : {4u, 8u, 16u, 32u}) {
+            const dim3 grid((unsigned)prop.multiProcessorCount * per_cu);
+            // copy: 4 GiB read + 4 GiB written per launch
+            hipLaunchKernelGGL(stream_copy<4>, grid, block, 0, 0, src, dst, bytes / 16);
+            HIP_TRY(hipEventRecord(t0));
+            for (int r = 0; r < 3; r++) hipLaunchKernelGGL(stream_copy<4>, grid, block, 0, 0, src, dst, bytes / 16);
+            HIP_TRY(hipEventRecord(t1));
+            HIP_TRY(hipEventSynchronize(t1));
+            HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+            best_copy = std::max(best_copy, 3.0 * 2.0 * (double)bytes / (ms * 1e-3) / 1e9);
+            // read: 4 GiB once per launch (far beyond every cache)
+            hipLaunchKernelGGL(stream_read<4>, grid, block, 0, 0, src, bytes / 16, 1, sink);
+            HIP_TRY(hipEventRecord(t0));
+            for (int r = 0; r < 3; r++) hipLaunchKernelGGL(stream_read<4>, grid, block, 0, 0, src, bytes / 16, 1, sink);
+            HIP_TRY(hipEventRecord(t1));
+            HIP_TRY(hipEventSynchronize(t1));
+            HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+            best_read = std::max(best_read, 3.0 * (double)bytes / (ms * 1e-3) / 1e9);
+            // 64 MiB read 40 times: resident in the Infinity Cache, not in the L2s
+            hipLaunchKernelGGL(stream_read<4>, grid, block, 0, 0, src, cached / 16, 2, sink);
+            HIP_TRY(hipEventRecord(t0));
+            hipLaunchKernelGGL(stream_read<4>, grid, block, 0, 0, src, cached / 16, 40, sink);
+            HIP_TRY(hipEventRecord(t1));
+            HIP_TRY(hipEventSynchronize(t1));
+            HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+            best_cache = std::max(best_cache, 40.0 * (double)cached / (ms * 1e-3) / 1e9);
+        }
+        if (copy_gbs) *copy_gbs = best_copy;
+        if (read_gbs) *read_gbs = best_read;
+        if (cache_read_gbs) *cache_read_gbs = best_cache;
+    }
+    HIP_TRY(hipGetLastError());
     rc = 0;
 fail:
     if (sink) (void)hipFree(sink);

@@ -254,6 +254,7 @@ int cgh_batch_tile(const cgh_batch* b, int d);
 int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int kernel, int use_graph, char* err, int errlen);
 void cgh_batch_free(cgh_batch* b);
 
+
 /* ------------------------------------------------------------------------ */
 /* Resident database + continuous-batching query server (no counterpart in   */
 /* the reference).                                                           */
@@ -327,6 +328,15 @@ void cgh_server_free(cgh_server* sv);
  * Any output pointer may be NULL.  Same role as tensorcore/peak.cu:8-54. */
 int cgh_measure_peaks(int device, double* i8_tops, double* fp4_tflops, double* popc_tops, double* hbm_gbs,
                       char* err, int errlen);
+
+/* Measured memory rates on `device` (GB/s): a streaming copy of 4 GiB
+ * (read + write counted), a read-only stream of 4 GiB, and a read-only
+ * stream over 64 MiB repeated 40 times (held in the Infinity Cache, larger
+ * than the L2s: the rate at which member lists and other reused data reach
+ * the CUs).  Any output pointer may be NULL.  No counterpart in the
+ * reference. */
+int cgh_measure_memory(int device, double* copy_gbs, double* read_gbs, double* cache_read_gbs, char* err,
+                       int errlen);
 
 /* Tile shape (rows = cols) used by cgh_dataset_upper for this dataset. */
 int cgh_dataset_tile(const cgh_dataset* ds, int kernel);
