@@ -34,7 +34,7 @@ EXPORTS = (
     "cgh_codec_create", "cgh_codec_info", "cgh_codec_members", "cgh_codec_encode", "cgh_codec_free",
     "cgh_db_create", "cgh_db_info", "cgh_db_free",
     "cgh_server_create", "cgh_server_slots", "cgh_server_submit", "cgh_server_wait", "cgh_server_free",
-    "cgh_measure_peaks", "cgh_measure_memory", "cgh_dataset_tile",
+    "cgh_measure_peaks", "cgh_measure_memory", "cgh_dataset_tile", "cgh_batch_run_noalloc",
 )
 
 PHASE_PRODUCT, PHASE_RARE = 1, 2
@@ -117,6 +117,7 @@ def load():
         "cgh_server_wait": (I, [P, C.c_long, P, P, P, C.POINTER(C.c_double), C.c_char_p, I]),
         "cgh_server_free": (None, [P]),
         "cgh_dataset_tile": (I, [P, I]),
+        "cgh_batch_run_noalloc": (I, [C.POINTER(P), I, I, P, P, I, C.c_char_p, I]),
         "cgh_measure_memory": (I, [I] + [C.POINTER(C.c_double)] * 3 + [C.c_char_p, I]),
     }
     for name, (res, args) in sig.items():
@@ -396,6 +397,15 @@ class Batch:
         err = _err()
         _check(load().cgh_batch_run(self.handle, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), kernel, int(graph), err,
                                     512), err)
+
+
+def batch_run_noalloc(datasets, S: int, out_ptr: int, stream_ptr: int = 0, kernel: int = 0):
+    """cgh_batch_run_noalloc: the whole batch of `datasets` in one call that
+    allocates no device memory (the layout of Batch(datasets, S))."""
+    arr = (C.c_void_p * len(datasets))(*[d.handle for d in datasets])
+    err = _err()
+    _check(load().cgh_batch_run_noalloc(arr, len(datasets), S, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), kernel,
+                                        err, 512), err)
 
 
 class QueryCodec:

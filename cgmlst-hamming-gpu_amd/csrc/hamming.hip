@@ -1805,6 +1805,236 @@ fail:
 }
 
 // ---------------------------------------------------------------------------
+// The batch without allocation (for callers that cannot allocate)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// What the tile generator and the queue kernels read, at the head of the
+// scratch array.  Per queue (0: 256-tiles, 1: 128-tiles) the datasets are
+// listed longest K first; list entry p owns tiles first[p] .. first[p+1]-1.
+struct NoAllocTable {
+    int count, S;
+    int ntiles[2];
+    int lists[2];
+    int order[2][CGH_NOALLOC_MAX_SETS];
+    long long first[2][CGH_NOALLOC_MAX_SETS + 1];
+    int tile[CGH_NOALLOC_MAX_SETS];
+    size_t outoff[CGH_NOALLOC_MAX_SETS];
+    TileJob jobs[CGH_NOALLOC_MAX_SETS];  // last: only `count` entries are stored
+};
+
+struct NoAllocScratch {
+    int next[64];  // queue counters (two used), padded to 256 bytes
+    NoAllocTable table;
+    int4 tiles[CGH_NOALLOC_MAX_TILES];  // the 256-tile queue, then the 128-tile queue
+};
+static_assert(sizeof(NoAllocScratch) <= CGH_NOALLOC_SCRATCH_BYTES, "scratch size in the header");
+
+}  // namespace
+
+// Device scratch of cgh_batch_run_noalloc: one copy per GPU, held until the
+// process ends.  Host code never uses this name as an address: the device's
+// own address of its copy comes from hipGetSymbolAddress (noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[CGH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+NoAllocScratch* scratch_at[MAX_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+hipError_t noalloc_scratch(int device, NoAllocScratch** out) {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    if (!scratch_at[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_at[device] = static_cast<NoAllocScratch*>(p);
+    }
+    *out = scratch_at[device];
+    return hipSuccess;
+}
+
+// Bytes carried into device memory by value as a kernel argument (at most
+// 4 KiB), so the table reaches the scratch in stream order with no host
+// staging buffer and no copy engine.
+constexpr unsigned PIECE = 2048;
+struct TablePiece {
+    unsigned offset, bytes;
+    unsigned char data[PIECE];
+};
+__global__ void __launch_bounds__(256) store_piece(unsigned char* dst, TablePiece p) {
+    for (unsigned k = threadIdx.x; k < p.bytes; k += 256) dst[p.offset + k] = p.data[k];
+}
+
+__global__ void zero_counters(int* next) {
+    if (threadIdx.x < 2) next[threadIdx.x] = 0;
+}
+
+// Tile t of queue `list` -> (dataset, stripe, I, J): the dataset from the
+// list's prefix sums, then the closed-form tile numbering of a one-dataset
+// launch (stripe by stripe, row-major; S is a multiple of the tile).
+__global__ void __launch_bounds__(256) noalloc_tiles(const NoAllocTable* __restrict__ tab, int list,
+                                                     int4* __restrict__ out) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= tab->ntiles[list]) return;
+    int lo = 0, hi = tab->lists[list] - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab->first[list][mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int d = tab->order[list][lo];
+    const int tl = tab->tile[d], n = tab->jobs[d].n, S = tab->S;
+    const long long local = t - tab->first[list][lo];
+    const long long q = S / tl, nbc = (n + tl - 1) / tl;
+    int slo = 0, shi = (n + S - 1) / S - 1;
+    while (slo < shi) {
+        const int mid = (slo + shi + 1) >> 1;
+        if (tiles_before(mid, q, nbc) <= local) slo = mid; else shi = mid - 1;
+    }
+    const long long rem = local - tiles_before(slo, q, nbc), tn = nbc - slo * q;
+    out[t] = make_int4(d, slo, (int)(rem / tn), (int)(rem % tn));
+}
+
+// Tiles of one dataset's upper-triangle layout in the closed-form numbering.
+long long layout_tiles(int n, int S, int tl) {
+    const int ns = (n + S - 1) / S;
+    const long long q = S / tl, nbc = (n + tl - 1) / tl;
+    const long long tm_last = (std::min(S, n - (ns - 1) * S) + tl - 1) / tl;
+    return tiles_before(ns - 1, q, nbc) + tm_last * (nbc - (long long)(ns - 1) * q);
+}
+
+}  // namespace
+
+extern "C" int cgh_batch_run_noalloc(cgh_dataset* const* sets, int count, int S, uint16_t* d_out, void* stream,
+                                     int kernel, char* err, int errlen) {
+    hipStream_t st = (hipStream_t)stream;
+    NoAllocScratch* scr = nullptr;
+    NoAllocTable tab{};  // host image (on the stack), stored into the scratch by kernel arguments
+    if (count < 1 || count > CGH_NOALLOC_MAX_SETS) {
+        set_err(err, errlen, "cgh_batch_run_noalloc: %d datasets (1..%d accepted)", count, CGH_NOALLOC_MAX_SETS);
+        return 1;
+    }
+    if (S < 256 || S % 256) {
+        set_err(err, errlen, "cgh_batch_run_noalloc: stripe height must be a positive multiple of 256 (got %d)", S);
+        return 1;
+    }
+    if (!valid_kernel(kernel)) {
+        set_err(err, errlen, "unknown kernel %d", kernel);
+        return 1;
+    }
+    if (((uintptr_t)d_out) & 15) {
+        set_err(err, errlen, "output buffer must be 16-byte aligned");
+        return 1;
+    }
+    const int device = sets[0]->device;
+    tab.count = count;
+    tab.S = S;
+    {
+        size_t off = 0;
+        long long cost[CGH_NOALLOC_MAX_SETS];
+        for (int d = 0; d < count; d++) {
+            const cgh_dataset* ds = sets[d];
+            if (ds->device != device) {
+                set_err(err, errlen, "dataset %d lives on device %d, not %d", d, ds->device, device);
+                return 1;
+            }
+            const int tl = batch_tile(ds->n), k = tl == 256 ? 0 : 1;
+            tab.tile[d] = tl;
+            tab.outoff[d] = off;
+            off += cgh_upper_elems(ds->n, S);
+            tab.jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL, ds->dO, ds->dP, ds->n, 0,
+                           (int)ds->n_pad, (int)ds->n_pad, ds->drstart, ds->cap};
+            cost[d] = (long long)(ds->kw1 + ds->kw2);
+            // insert into queue k's list, longest K first
+            int p = tab.lists[k]++;
+            while (p > 0 && cost[tab.order[k][p - 1]] < cost[d]) {
+                tab.order[k][p] = tab.order[k][p - 1];
+                p--;
+            }
+            tab.order[k][p] = d;
+        }
+        long long total = 0;
+        for (int k = 0; k < 2; k++) {
+            tab.first[k][0] = 0;
+            for (int p = 0; p < tab.lists[k]; p++) {
+                const int d = tab.order[k][p];
+                tab.first[k][p + 1] = tab.first[k][p] + layout_tiles(sets[d]->n, S, tab.tile[d]);
+            }
+            total += tab.first[k][tab.lists[k]];
+        }
+        if (total > CGH_NOALLOC_MAX_TILES) {
+            set_err(err, errlen, "cgh_batch_run_noalloc: %lld tiles (at most %d)", total, CGH_NOALLOC_MAX_TILES);
+            return 1;
+        }
+        tab.ntiles[0] = (int)tab.first[0][tab.lists[0]];
+        tab.ntiles[1] = (int)tab.first[1][tab.lists[1]];
+    }
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(noalloc_scratch(device, &scr));
+    {
+        // the table up to its last used job, then the counters, the tile
+        // lists and the launches of cgh_batch_run (rare passes on `stream`)
+        const size_t bytes = offsetof(NoAllocTable, jobs) + (size_t)count * sizeof(TileJob);
+        unsigned char* dst = reinterpret_cast<unsigned char*>(&scr->table);
+        for (size_t o = 0; o < bytes; o += PIECE) {
+            TablePiece piece;
+            piece.offset = (unsigned)o;
+            piece.bytes = (unsigned)std::min<size_t>(PIECE, bytes - o);
+            memcpy(piece.data, reinterpret_cast<const unsigned char*>(&tab) + o, piece.bytes);
+            hipLaunchKernelGGL(store_piece, dim3(1), dim3(256), 0, st, dst, piece);
+        }
+        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, st, scr->next);
+        int4* lists[2] = {scr->tiles, scr->tiles + tab.ntiles[0]};
+        for (int k = 0; k < 2; k++)
+            if (tab.ntiles[k])
+                hipLaunchKernelGGL(noalloc_tiles, dim3((unsigned)((tab.ntiles[k] + 255) / 256)), dim3(256), 0, st,
+                                   &scr->table, k, lists[k]);
+        HIP_TRY(hipGetLastError());
+        const int kern = resolve_kernel(kernel);
+        const unsigned cus = (unsigned)rare_grid(1 << 30);
+        const TileJob* jobs = scr->table.jobs;
+        const size_t* outoff = scr->table.outoff;
+        if (tab.ntiles[0]) {
+            if (kern == CGH_KERNEL_VALU_POPC)
+                hipLaunchKernelGGL((valu_tile_queue<256>), dim3(cus * 2), dim3(256), 0, st, jobs, d_out, outoff, lists[0],
+                                   tab.ntiles[0], scr->next);
+            else if (kern == CGH_KERNEL_MFMA_FP4)
+                hipLaunchKernelGGL((mfma_tile_queue<256, 128, 64, true>), dim3(cus), dim3(512), 0, st, jobs, d_out,
+                                   outoff, lists[0], tab.ntiles[0], scr->next);
+            else
+                hipLaunchKernelGGL((mfma_tile_queue<256, 128, 64, false>), dim3(cus), dim3(512), 0, st, jobs, d_out,
+                                   outoff, lists[0], tab.ntiles[0], scr->next);
+        }
+        if (tab.ntiles[1]) {
+            if (kern == CGH_KERNEL_VALU_POPC)
+                hipLaunchKernelGGL((valu_tile_queue<128>), dim3(cus * 2), dim3(256), 0, st, jobs, d_out, outoff,
+                                   lists[1], tab.ntiles[1], scr->next + 1);
+            else if (kern == CGH_KERNEL_MFMA_FP4)
+                hipLaunchKernelGGL((mfma_tile_queue<128, 64, 64, true>), dim3(cus * 2), dim3(256), 0, st, jobs, d_out,
+                                   outoff, lists[1], tab.ntiles[1], scr->next + 1);
+            else
+                hipLaunchKernelGGL((mfma_tile_queue<128, 64, 64, false>), dim3(cus * 2), dim3(256), 0, st, jobs, d_out,
+                                   outoff, lists[1], tab.ntiles[1], scr->next + 1);
+        }
+        HIP_TRY(hipGetLastError());
+        for (int d = 0; d < count; d++) {
+            const cgh_dataset* ds = sets[d];
+            if (!ds->n_rare || ds->n < 2) continue;
+            RareJob jb = rare_job(d_out + tab.outoff[d], ds->n, 0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16,
+                                  ds->cap, lanes_per_range(ds->n_rare, ds->n_rentries));
+            HIP_TRY(launch_rare(jb, rare_layout(jb, ds->n), st));
+        }
+    }
+    return 0;
+fail:
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
 // Resident database and a continuous-batching query server
 // ---------------------------------------------------------------------------
 

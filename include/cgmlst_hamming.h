@@ -254,6 +254,24 @@ int cgh_batch_tile(const cgh_batch* b, int d);
 int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int kernel, int use_graph, char* err, int errlen);
 void cgh_batch_free(cgh_batch* b);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cgh_batch_create + cgh_batch_run (eager) in one call
+ * that allocates nothing.  The job table, the two tile queues and their
+ * counters live in a static device array of CGH_NOALLOC_SCRATCH_BYTES that
+ * every GPU the library runs on holds until the process ends; every call on
+ * a GPU rewrites its copy from the start, so calls that overlap on one GPU
+ * race (serialise them, e.g. on one stream).  At most CGH_NOALLOC_MAX_SETS
+ * datasets and CGH_NOALLOC_MAX_TILES tiles (the triangle tiles of every
+ * dataset at its batch tile edge, 256 from 4,096 rows, else 128); S a
+ * positive multiple of 256.  d_out: 16-byte aligned, dataset d's stripe
+ * layout at element sum_{e < d} cgh_upper_elems(n_e, S) (cgh_batch_create's
+ * layout); all launches, rare passes included, go to `stream`. */
+#define CGH_NOALLOC_MAX_SETS 256
+#define CGH_NOALLOC_MAX_TILES (1 << 20)
+#define CGH_NOALLOC_SCRATCH_BYTES ((size_t)17 << 20)
+int cgh_batch_run_noalloc(cgh_dataset* const* sets, int count, int S, uint16_t* d_out, void* stream, int kernel,
+                          char* err, int errlen);
 
 /* ------------------------------------------------------------------------ */
 /* Resident database + continuous-batching query server (no counterpart in   */
