@@ -314,7 +314,7 @@ extern "C" int cgh_dataset_encode(int device, const uint64_t* calls, int n, int 
         HIP_TRY(hipMemcpy(dgid, gid.data(), ncells * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMalloc(&dgs, gstart.size() * 4));
         HIP_TRY(hipMemcpy(dgs, gstart.data(), gstart.size() * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMalloc(&ds.dgmem, members * (ds.m16 ? 2 : 4)));
+        HIP_TRY(hipMalloc(&ds.dgmem, cghi::member_array_bytes(members, ds.m16)));
         HIP_TRY(hipMalloc(&drc, ((size_t)n + 1) * 4));
         HIP_TRY(hipMemset(drc, 0, ((size_t)n + 1) * 4));
         if (ds.m16)

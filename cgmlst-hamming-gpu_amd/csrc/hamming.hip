@@ -86,6 +86,10 @@ struct TileJob {
     // subtraction; the epilogue caps the others (rstart[i + 1] == rstart[i]).
     const uint32_t* rstart;
     int cap;
+    // Words of O and P that can hold a set bit (ceil(D / 32), ceil(L / 32));
+    // the padding after them is zero, and the VALU form skips its planes.
+    // 0 = unknown (every word).
+    int kv1, kv2;
 };
 
 // Cell value of row i (global) in the epilogue, capped unless the rare pass
@@ -491,6 +495,8 @@ __device__ __forceinline__ void scalarize(TileJob& j) {
     j.ldB = __builtin_amdgcn_readfirstlane(j.ldB);
     j.rstart = uniform_ptr(j.rstart);
     j.cap = __builtin_amdgcn_readfirstlane(j.cap);
+    j.kv1 = __builtin_amdgcn_readfirstlane(j.kv1);
+    j.kv2 = __builtin_amdgcn_readfirstlane(j.kv2);
 }
 
 // Persistent form for many datasets: each workgroup pulls tiles (job,
@@ -588,8 +594,11 @@ __device__ __forceinline__ void valu_sub(const TileJob a, int sr0, int rt0, int 
 #pragma unroll
                 for (int j = 0; j < 8; j++) acc[i][j] = 0u - acc[i][j];
         }
+        // planes of this slab that hold data (the rest is zero padding)
+        const int rem = c < nc1 ? (a.kv1 ? a.kv1 : a.kw1) - c * KC : (a.kv2 ? a.kv2 : a.kw2) - (c - nc1) * KC;
+        const int npl = min(KC / 4, max(0, (rem + 3) / 4));
 #pragma unroll 1
-        for (int pl = 0; pl < KC / 4; pl++) {  // not unrolled: 16 rows x 4 words live at a time
+        for (int pl = 0; pl < npl; pl++) {  // not unrolled: 16 rows x 4 words live at a time
             uint4 ar[8], br[8];
 #pragma unroll
             for (int i = 0; i < 8; i++) ar[i] = st[buf].a[ty + 16 * i][pl];
@@ -709,7 +718,7 @@ __global__ void reset_counter(int* c) { *c = 0; }
 constexpr int RCW = 65536;  // u16 cells per chunk (128 KiB of LDS: one workgroup per CU)
 constexpr int RNT = 1024;   // threads per workgroup = ranges per batch
 constexpr int RQ = (RCW / 8 + 1 + RNT - 1) / RNT;  // 16-byte words staged per thread
-constexpr int RUN = 8;      // partners per lane per pass
+constexpr int MPL = 8;      // members per lane per unit of a range walk (one 16-byte word of u16 members)
 // Lanes per range: about 0.4 x the mean range length (partners per range),
 // a power of two in [4, 16]: short ranges leave fewer lanes idle, long ones
 // keep fewer ranges in flight.
@@ -843,7 +852,7 @@ __device__ __forceinline__ void rare_stage_land(const RareItem& it, const v4i (&
 // Row i's partners are ranges of the ascending group member lists (rentry):
 // a batch of up to RNT ranges is read (and clipped to the chunk) by all
 // threads at once into LDS; then each rsg-lane subgroup takes ranges from
-// LDS, its lanes reading consecutive members (RUN loads in flight per lane)
+// LDS, its lanes reading consecutive members (MPL members in flight per lane)
 // and applying them with LDS atomics.  The first barrier also orders the
 // staging stores before the atomics.  MASK: ranges in arbitrary order (not
 // clipped); a member outside the item's columns [j0, j1) is skipped.
@@ -854,7 +863,6 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
     const uint64_t* __restrict__ rentry = jb.rentry;
     uint32_t* ebeg = sh.ebeg;
     uint32_t* eend = sh.eend;
-    uint32_t* cells = reinterpret_cast<uint32_t*>(sh.cells4);
     const int tid = threadIdx.x;
     for (int i = it.i0; i < it.i1; i++) {
         const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
@@ -886,33 +894,50 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
             // Subgroup sub starts with range sub and then takes the next free
             // one from sh.next, so a few long ranges do not leave the other
             // subgroups idle at the barrier.  A range is walked in units of
-            // rsg * RUN members, and the members of the next unit are loaded
-            // while the current unit's atomics are issued, so the member-load
-            // latency overlaps the LDS work instead of adding to it.  Loads and
-            // atomics are issued for every slot: past the unit end a lane
-            // reloads the last member and adds 0 to a word of its own (no
-            // branches, so the load waits are fixed counts).
+            // rsg x MPL members read as whole 16-byte words (lane sl takes
+            // words sl, sl + rsg, ... of the unit, so each load instruction
+            // of a subgroup reads consecutive words), the unit starting at
+            // the word that holds the range's first member.  The next unit's
+            // words are loaded while the current unit's atomics are issued,
+            // so the member-load latency overlaps the LDS work.  Loads and
+            // atomics are issued for every slot: a member outside the range
+            // goes to a word of the lane's own (no branches, so the load
+            // waits are fixed counts), and a word past the range's last one
+            // reads that last word instead (never past the member array).
+            //
+            // A partner j of row i is the u16 cell at LDS offset o = bo + j:
+            // the 32-bit word at byte 2o & ~3, low half when o is even.
+            // Subtracting 1 from that cell adds -1 << (16 (o & 1)) to the word
+            // (a cell never drops below its final value, so no borrow crosses
+            // the halves); the shift amount is taken from o << 4, whose low
+            // five bits are 16 (o & 1) (a 32-bit shift reads only those).
+            constexpr int PER = 16 / (int)sizeof(M);  // members per 16-byte word
+            constexpr int WPL = MPL / PER;            // words per lane per unit
+            const v4i* __restrict__ words = reinterpret_cast<const v4i*>(gmem);
             const int rsg = jb.rsg, sub = tid / rsg, sl = tid % rsg, leader = (tid & 63) - sl;
-            const uint32_t own = (uint32_t)(reinterpret_cast<uint32_t*>(sh.spare) - cells) + (tid & 63);
+            char* const lds = reinterpret_cast<char*>(sh.cells4);
+            const uint32_t own = (uint32_t)(reinterpret_cast<char*>(&sh.spare[tid & 63]) - lds);
             const uint32_t bo = (uint32_t)(base - it.a0);  // LDS u16 offset of cell (i, 0), mod 2^32
-            // current unit: members [cp0, min(cp0 + step, cend)) of its range
-            const uint32_t step = (uint32_t)(rsg * RUN);
+            const uint32_t ustep = (uint32_t)(rsg * WPL);         // words per unit
+            auto load_unit = [&](v4i (&w)[WPL], uint32_t uw, uint32_t end) {
+                const uint32_t last = (max(end, 1u) - 1) / PER;
+#pragma unroll
+                for (int m = 0; m < WPL; m++) w[m] = words[min(uw + sl + m * rsg, last)];
+            };
+            // current unit: words [uw, uw + ustep) of the range [cp0, cend)
             uint32_t cp0 = 0, cend = 0;
             if (sub < nb) {
                 cp0 = ebeg[sub];
                 cend = eend[sub];
             }
-            uint32_t ja[RUN];
-#pragma unroll
-            for (int u = 0; u < RUN; u++)
-                ja[u] = (uint32_t)gmem[min(cp0 + sl + u * rsg, max(cend, 1u) - 1)];
-            for (;;) {
-                uint32_t np0, nend;
+            uint32_t uw = cp0 / PER;
+            // One unit: find the next unit (the rest of this range, else the
+            // next free range), load its words into `nxt`, then issue the
+            // atomics of `cur`.  Returns false when no unit follows.
+            auto unit = [&](const v4i (&cur)[WPL], v4i (&nxt)[WPL]) -> bool {
+                uint32_t np0 = cp0, nend = cend, nuw = uw + ustep;
                 bool more = true;
-                if (cp0 + step < cend) {
-                    np0 = cp0 + step;
-                    nend = cend;
-                } else {
+                if (nuw * PER >= cend) {  // this range ends in the current unit: take the next one
                     int nx = 0;
                     if (sl == 0) nx = (int)atomicAdd(&sh.next, 1u);
                     nx = __shfl(nx, leader, 64);
@@ -924,26 +949,41 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
                     } else {
                         more = false;
                     }
+                    nuw = np0 / PER;
                 }
-                uint32_t jn[RUN];
+                load_unit(nxt, nuw, nend);
+                __builtin_amdgcn_sched_barrier(0);  // the next unit's loads go out before this unit's atomics
+                const uint32_t len = cend - cp0;
 #pragma unroll
-                for (int u = 0; u < RUN; u++)
-                    jn[u] = (uint32_t)gmem[min(np0 + sl + u * rsg, max(nend, 1u) - 1)];
-                if (cp0 < cend) {
+                for (int m = 0; m < WPL; m++) {
+                    // position of the word's first member relative to the range start
+                    const uint32_t q0 = (uw + sl + m * rsg) * PER - cp0;
 #pragma unroll
-                    for (int u = 0; u < RUN; u++) {
-                        const bool in = cp0 + sl + u * rsg < cend &&
-                                        (!MASK || (ja[u] >= (uint32_t)it.j0 && ja[u] < (uint32_t)it.j1));
-                        const uint32_t o = bo + ja[u];
-                        atomicSub(&cells[in ? (o >> 1) : own], in ? ((o & 1) ? 0x10000u : 1u) : 0u);
+                    for (int h = 0; h < PER; h++) {
+                        const uint32_t x = (uint32_t)cur[m][PER == 8 ? h >> 1 : h];
+                        const uint32_t j = PER == 8 ? __builtin_amdgcn_ubfe(x, (h & 1) * 16, 16) : x;
+                        bool in = q0 + h < len;
+                        if constexpr (MASK) in = in && j >= (uint32_t)it.j0 && j < (uint32_t)it.j1;
+                        const uint32_t o = bo + j;
+                        const uint32_t addr = in ? ((o << 1) & ~3u) : own;
+                        atomicAdd(reinterpret_cast<uint32_t*>(lds + addr), ~0u << ((o << 4) & 31u));
                     }
                 }
-                if (!more) break;
                 cp0 = np0;
                 cend = nend;
-#pragma unroll
-                for (int u = 0; u < RUN; u++) ja[u] = jn[u];
+                uw = nuw;
+                return more;
+            };
+            // two register sets in turn, so no unit's words are copied (a
+            // copy would wait for the loads it was meant to overlap)
+            v4i wa[WPL], wb[WPL];
+            load_unit(wa, uw, cend);
+            while (unit(wa, wb) && unit(wb, wa)) {
             }
+            // both sets are used on the way out too, so the loads of a next
+            // unit stay ahead of the atomics instead of sinking onto the
+            // loop's back edge behind them
+            asm volatile("" : : "v"(wa[0]), "v"(wb[0]));
             __syncthreads();  // ebeg / eend / next are rewritten by the next batch
         }
     }
@@ -1153,7 +1193,7 @@ int kc_words(size_t kw) { return (int)up(kw, (size_t)KC); }
 hipError_t upload_members(const uint32_t* mem, size_t count, bool m16, void** out) {
     hipError_t e;
     *out = NULL;
-    if ((e = hipMalloc(out, std::max<size_t>(count, 1) * (m16 ? 2 : 4)))) return e;
+    if ((e = hipMalloc(out, cghi::member_array_bytes(count, m16)))) return e;
     if (!count) return hipSuccess;
     if (m16) {
         std::vector<uint16_t> h(count);
@@ -1171,12 +1211,12 @@ hipError_t upload_members(const uint32_t* mem, size_t count, bool m16, void** ou
 
 hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int ld, int n,
                         int r0, int S, int ns, uint16_t* out, hipStream_t st, const uint32_t* rstart = NULL,
-                        int cap = 0) {
+                        int cap = 0, int kv1 = 0, int kv2 = 0) {
     if (ns > 1 && S % tile) {
         for (int s = 0; s < ns; s++) {
             const hipError_t e = launch_tiles(kernel, tile, O, kw1, P, kw2, ld, n, r0 + s * S, S, 1,
                                               out + stripe_off((size_t)(n - r0), (size_t)S, (size_t)s), st, rstart,
-                                              cap);
+                                              cap, kv1, kv2);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -1187,6 +1227,7 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
     a.job.OB = O; a.job.PB = P; a.job.nB = n; a.job.rect = 0;
     a.job.ldA = ld; a.job.ldB = ld;
     a.job.rstart = rstart; a.job.cap = cap;
+    a.job.kv1 = kv1; a.job.kv2 = kv2;
     a.ns = ns;
     a.nbc = (n - r0 + tile - 1) / tile;
     a.q = S / tile;
@@ -1456,6 +1497,9 @@ namespace {
 int ds_kernel(const cgh_dataset* ds, int kernel) {
     return kernel == CGH_KERNEL_AUTO && ds->tuned_kernel ? ds->tuned_kernel : resolve_kernel(kernel);
 }
+// Words of O and P that can hold a set bit: ceil(D / 32), ceil(L / 32).
+int ds_kv1(const cgh_dataset* ds) { return (int)((ds->D + 31) / 32); }
+int ds_kv2(const cgh_dataset* ds) { return (int)(((size_t)ds->L + 31) / 32); }
 int ds_tile(const cgh_dataset* ds, int kernel) {
     return ds->tuned_tile && ds_kernel(ds, kernel) == ds->tuned_kernel ? ds->tuned_tile : pick_tile(ds->n);
 }
@@ -1489,11 +1533,12 @@ extern "C" int cgh_dataset_autotune(cgh_dataset* ds, int S, int* kernel_out, int
     for (int k : kernels)
         for (int t : tiles) {
             float ms = 0.f;
-            HIP_TRY(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf, st));
+            HIP_TRY(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf, st,
+                                 NULL, 0, ds_kv1(ds), ds_kv2(ds)));
             HIP_TRY(hipEventRecord(e0, st));
             for (int rep = 0; rep < 3; rep++)
                 HIP_TRY(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf,
-                                st));
+                                     st, NULL, 0, ds_kv1(ds), ds_kv2(ds)));
             HIP_TRY(hipEventRecord(e1, st));
             HIP_TRY(hipEventSynchronize(e1));
             HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
@@ -1540,7 +1585,7 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
     HIP_TRY(hipSetDevice(ds->device));
     if (phases & CGH_PHASE_PRODUCT)
         HIP_TRY(launch_tiles(ds_kernel(ds, kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
-                         S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap));
+                             S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds)));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         RareJob jb = rare_job(d_out, n, 0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap,
                               lanes_per_range(ds->n_rare, ds->n_rentries));
@@ -1641,7 +1686,7 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
         b->offs.push_back(b->total);
         b->total += cgh_upper_elems(ds->n, S);
         jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL, ds->dO, ds->dP, ds->n, 0,
-                   (int)ds->n_pad, (int)ds->n_pad, ds->drstart, ds->cap};
+                   (int)ds->n_pad, (int)ds->n_pad, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds)};
         const int k = tl == 256 ? 0 : 1;
         const long long cost = (long long)(ds->kw1 + ds->kw2);
         for (int s = 0, r0 = 0; r0 < ds->n; s++, r0 += S) {
@@ -1947,7 +1992,7 @@ extern "C" int cgh_batch_run_noalloc(cgh_dataset* const* sets, int count, int S,
             tab.outoff[d] = off;
             off += cgh_upper_elems(ds->n, S);
             tab.jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL, ds->dO, ds->dP, ds->n, 0,
-                           (int)ds->n_pad, (int)ds->n_pad, ds->drstart, ds->cap};
+                           (int)ds->n_pad, (int)ds->n_pad, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds)};
             cost[d] = (long long)(ds->kw1 + ds->kw2);
             // insert into queue k's list, longest K first
             int p = tab.lists[k]++;
@@ -2263,7 +2308,7 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     const int tile = q > 128 ? 256 : 128;
     TileArgs a;
     a.job = {sl.dO, sl.dP, (int)db->ds->kw1, (int)db->ds->kw2, q, 0, q, sl.ddist, db->ds->dO, db->ds->dP, db->n, 1,
-             q_pad, (int)db->ds->n_pad, NULL, 0};
+             q_pad, (int)db->ds->n_pad, NULL, 0, ds_kv1(db->ds), ds_kv2(db->ds)};
     a.ns = 1;
     a.q = 0;
     a.nbc = (db->n + tile - 1) / tile;
@@ -2655,7 +2700,7 @@ struct CsrRanges {
         if (rentry) (void)hipFree(rentry);
     }
     static size_t device_bytes(int n, size_t n_rare) {
-        return std::max<size_t>(n_rare, 1) * 4 + ((size_t)n + 1) * 4 + (size_t)n * 8;
+        return cghi::member_array_bytes(n_rare, false) + ((size_t)n + 1) * 4 + (size_t)n * 8;
     }
     hipError_t upload(int n, const uint32_t* start, const uint32_t* js, size_t n_rare) {
         std::vector<uint32_t> rs((size_t)n + 1);
@@ -2667,7 +2712,7 @@ struct CsrRanges {
         }
         rs[(size_t)n] = (uint32_t)re.size();
         entries = re.size();
-        hipError_t e = hipMalloc(&members, std::max<size_t>(n_rare, 1) * 4);
+        hipError_t e = hipMalloc(&members, cghi::member_array_bytes(n_rare, false));
         if (!e) e = hipMalloc(&rstart, rs.size() * 4);
         if (!e) e = hipMalloc(&rentry, std::max<size_t>(entries, 1) * 8);
         if (!e && n_rare) e = hipMemcpy(members, js, n_rare * 4, hipMemcpyHostToDevice);
@@ -2775,7 +2820,7 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
     {
         auto produce = [&](int r0, int rows, uint16_t* dst, hipStream_t st) -> hipError_t {
             hipError_t e = launch_tiles(kernel, tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n,
-                                        r0, S, 1, dst, st, ds->drstart, ds->cap);
+                                        r0, S, 1, dst, st, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds));
             if (!e && ds->n_rare && n - r0 > 1) {
                 RareJob jb = rare_job(dst, n, r0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap, rsg);
                 e = launch_rare(jb, rare_layout(jb, rows), st);

@@ -39,6 +39,13 @@ bool hip_failed(hipError_t e, const char* call, char* err, int errlen);
         if (cghi::hip_failed((x), #x, err, errlen)) goto fail;         \
     } while (0)
 
+// Bytes of a device array of `count` group members (u16 when m16, else u32):
+// whole 16-byte words, at least one, since the rare pass reads members 16
+// bytes at a time and its last word of a list may extend past the list.
+inline size_t member_array_bytes(size_t count, bool m16) {
+    return ((count * (m16 ? 2 : 4) + 15) / 16 + 1) * 16;
+}
+
 // Device words per row for host words kw (rounded up to the product slab).
 int device_words(size_t kw);
 
