@@ -1,12 +1,24 @@
 #!/usr/bin/env python3
-"""Benchmark: exact u16 upper-triangle Hamming matrix of the long-tail
-63,005 x 1,142 profile set on one MI355X.
+"""Benchmark: exact u16 allelic Hamming distances on one MI355X.
 
-A step is one pass of the hot path over the whole upper triangle with the
-operands already resident in HBM: the MFMA binary product with its fused
-u16 epilogue (every stripe in one launch), then the rare-pair subtraction.
-Output: one JSON line on stdout (metric, value = Gpairs/s, roofline of the
-product kernel, CPU direct-compare baseline timed on this host).
+Headline: the whole upper triangle of the long-tail 63,005 x 1,142 profile
+set (the metric's configuration), operands resident in HBM.  A step is one
+pass of the hot path: the MFMA binary product with its fused u16 epilogue
+(every stripe in one launch), then the row-local rare-pair pass, output
+resident in HBM.  The same JSON line carries
+  roofline / rooflines   the product (MFMA-bound) and the rare pass (bytes
+                         that must reach HBM; its measured limiter named)
+  batch                  64 uneven datasets: one batched persistent launch
+                         (HIP graph replay) against one launch pair per
+                         dataset, identical output checked
+  serve                  a resident 200,000 x 1,142 database answering a
+                         Poisson stream of 1-256-query requests: p50 / p99
+                         latency and saturated Gpairs/s of the continuous-
+                         batching server against plain per-request service,
+                         identical results checked
+  cpu_baseline           the oracle's direct compare built for this host's
+                         CPU, timed on the 2,000 x 1,142 set in full and on
+                         a sample of the headline set
 """
 import argparse
 import importlib.util
@@ -98,122 +110,229 @@ def summarize_counters(dirs, tau, kern, out_path):
     print(json.dumps(rec["kernels"]), flush=True)
 
 
-def cpu_baseline(X, flat, cgh, target_s=15.0):
-    """Direct compare (the oracle's OpenMP C brute force) on the first rows of
-    the same workload, timed on this host; the same rows of the GPU output
-    are checked against it."""
-    import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+def counter_traffic(tau, kern):
+    """HBM-side bytes per launch of the product and rare kernels from the
+    committed counter passes, when they cover this configuration and this
+    kernel source; else None for both."""
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_long_tail_63005x1142.json")
+    if not os.path.exists(tpath):
+        return {"product": None, "rare": None}
+    t_rec = json.load(open(tpath))
+    if ((t_rec["n"], t_rec["L"], t_rec["tau"], t_rec["stripe"], t_rec.get("product_kernel")) != (N, L, tau, S, kern)
+            or t_rec.get("kernel_source_sha256") != kernel_source_sha()):
+        return {"product": None, "rare": None}
+    return {k: t_rec["kernels"][k]["traffic_bytes"] for k in ("product", "rare")}
 
-    n = X.shape[0]
-    probe = 256
+
+def headline(args, cgh, wl, torch):
+    """The 63,005 x 1,142 long-tail set: product + rare pass per step."""
     t = time.perf_counter()
-    oracle.upper_block(X, 0, probe)
-    dt = time.perf_counter() - t
-    per_row = dt / probe
-    rows = int(min(n // 2, max(probe, target_s / max(per_row, 1e-9))))
+    calls = wl.long_tail(N, L, seed=2)
+    X = cgh.dense_ranks(calls)
+    del calls
+    log(f"workload long_tail {N}x{L} generated + ranked in {time.perf_counter() - t:.1f}s")
+    tau = args.tau or cgh.choose_tau(X, cgh.upper_elems(N, S) * 2, cgh.free_vram(0), log=log)
     t = time.perf_counter()
-    ref = oracle.upper_block(X, 0, rows)
-    dt = time.perf_counter() - t
-    pairs = int(ref.size)
-    # the same cells from the GPU's stripe layout
-    parts = []
-    for i in range(rows):
-        s = i // S
-        r0 = s * S
-        off = cgh.stripe_offset(n, S, s) + (i - r0) * (n - r0)
-        parts.append(flat[off + (i + 1 - r0): off + (n - r0)])
-    exact = bool(np.array_equal(np.concatenate(parts), ref))
-    return {
-        "value": pairs / dt / 1e9, "unit": "Gpairs/s", "cores": oracle.num_threads(), "kind": "port",
-        "sample": f"rows 0..{rows - 1} of the 63,005 x 1,142 set vs all later rows ({pairs} pairs, {dt:.1f} s), "
-                  f"OpenMP C direct compare sum_k [a!=0][b!=0][a!=b]",
-        "sample_matches_gpu": exact,
+    enc = cgh.Encoded(X, tau, 256)
+    e = enc.enc
+    D, R = int(e.D), int(e.n_rare)
+    members, entries = int(e.n_members), int(e.n_rentries)
+    log(f"encoded tau={tau} D={D} rare_pairs={R} kw1={e.kw1} kw2={e.kw2} in {time.perf_counter() - t:.1f}s")
+    ds = cgh.Dataset(enc, 0)
+    del enc
+    out = torch.empty(cgh.upper_elems(N, S), dtype=torch.int16, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def step(ev=None):
+        if ev:
+            ev[0].record(stream)
+        ds.phase(S, out.data_ptr(), cgh.PHASE_PRODUCT, sp, args.kernel)
+        if ev:
+            ev[1].record(stream)
+        ds.phase(S, out.data_ptr(), cgh.PHASE_RARE, sp)
+        if ev:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prod_ms = sum(evs[k][0].elapsed_time(evs[k][1]) for k in range(args.steps)) / args.steps
+    rare_ms = sum(evs[k][1].elapsed_time(evs[k][2]) for k in range(args.steps)) / args.steps
+    ms_step = wall * 1e3 / args.steps
+    pairs = N * (N - 1) // 2
+    kern = 2 if args.kernel == 0 else args.kernel  # the automatic choice is fp4
+    peak, punit, dtype = PEAKS[kern]
+    log(f"step {ms_step:.2f} ms (product {prod_ms:.2f} ms, rare {rare_ms:.2f} ms)")
+
+    measured = cgh.measure_peaks(0)
+    memory = cgh.measure_memory(0)
+    log("measured peaks: " + ", ".join(f"{k} {v:.0f}" for k, v in measured.items()) + "; memory: " +
+        ", ".join(f"{k} {v:.0f}" for k, v in memory.items()))
+    traffic = counter_traffic(tau, kern)
+
+    # product: 2 (D + L) ops per pair (a multiply-add per bit pair)
+    achieved = 2.0 * pairs * (D + L) / (prod_ms * 1e-3) / 1e12
+    # rare pass: the bytes that must reach HBM are the output cells read and
+    # written once (4 B per pair) and the member lists and range index read
+    # once; the 2 B member index read per rare pair comes from the caches
+    m_bytes = 2 if N <= 65535 else 4
+    rare_hbm = 4.0 * pairs + m_bytes * members + 8.0 * entries + 4.0 * (N + 1)
+    rare_gbs = rare_hbm / (rare_ms * 1e-3) / 1e9
+    rooflines = {
+        "product": {"bound": "valu" if kern == 3 else "mfma", "achieved": achieved, "peak": peak, "unit": punit,
+                    "frac": achieved / peak, "traffic": traffic["product"],
+                    "kernel": ("valu_tile" if kern == 3 else "mfma_tile") +
+                              " (binary product + u16 epilogue, one launch per step)",
+                    "work": "2*(D+L) ops per pair (multiply-add per bit pair), pairs = n(n-1)/2, D unpadded",
+                    "ms": prod_ms, "frac_of_measured": achieved / measured[MEASURED_PEAK_KEY[kern]]},
+        "rare": {"bound": "hbm", "achieved": rare_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": rare_gbs / HBM_PEAK_GBS, "traffic": traffic["rare"],
+                 "kernel": "sub_rare_rows (rare equal pairs subtracted in LDS, one launch per step)",
+                 "work": "4 B per pair (u16 cell read + write) + member lists and range index once "
+                         f"({members} members, {entries} ranges); the {R} member reads (2 B each) hit the caches",
+                 "ms": rare_ms, "frac_of_measured_copy": rare_gbs / memory["copy_gbs"],
+                 "rare_pairs_per_s": R / (rare_ms * 1e-3),
+                 "limiter": "VALU issue: ~8 vector instructions per rare pair (LDS address, half-word value, "
+                            "range mask) on 16 waves per CU; see profiles/sub_rare_rows_counters_*.md"},
     }
+    if traffic["rare"]:  # L2 <-> fabric bytes the counters saw, against the cache-resident read rate
+        rooflines["rare"]["l2_fabric"] = {"achieved": traffic["rare"] / (rare_ms * 1e-3) / 1e9,
+                                          "ceiling": memory["cache_read_gbs"], "unit": "GB/s"}
+    dominant = "product" if prod_ms >= rare_ms else "rare"
+    rec = {
+        "metric": "Gpairs/s exact H at 63k x 1142 (u16 upper triangle, device-resident)",
+        "value": pairs / (ms_step * 1e-3) / 1e9,
+        "unit": "Gpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded long-tail Zipf(1.1) allele calls, Beta(0.5,15) per-sample missing)",
+        "config": {"workload": "long_tail", "n": N, "L": L, "tau": tau, "stripe": S, "layout": "upper-tri u16",
+                   "one_hot_cols": D, "rare_pairs": R, "tile": ds.tile(), "kernel": kern},
+        "roofline": rooflines[dominant],
+        "rooflines": rooflines,
+        "phases_ms": {"product": prod_ms, "rare": rare_ms},
+        "measured_peaks": dict(measured, **{"read_gbs": memory["read_gbs"], "cache_read_gbs": memory["cache_read_gbs"],
+                                            "copy_gbs": max(measured["copy_gbs"], memory["copy_gbs"])}),
+        "frac_of_measured_peak": achieved / measured[MEASURED_PEAK_KEY[kern]],
+    }
+    flat = out.cpu().numpy().view("uint16")
+    del out
+    ds.close()
+    return rec, X, flat
 
 
-def run_batch(args, cgh, wl, torch):
+def batch_record(args, cgh, wl, torch):
     """64 independent datasets (n ~ U[500, 20000], L ~ U[800, 3000], Zipf(1.2)
-    long tail, 2% missing), each with its own cost-model tau: the batched
-    persistent launch (HIP graph) against one launch pair per dataset, same
-    output arena, outputs compared for equality."""
-    import numpy as np
+    long tail, 2% missing, ids relabelled to 32 / 64 bits), each with its own
+    cost-model tau: the batched persistent launch replayed from its HIP graph
+    against one launch pair per dataset (each with its autotuned kernel and
+    tile), the same output arena, outputs compared cell for cell."""
+    from collections import Counter
+    from concurrent.futures import ThreadPoolExecutor
     Sb = 4096
     shapes = wl.batch_shapes(64, seed=4)
     t = time.perf_counter()
-    sets, pairs, taus, tuned = [], 0, [], []
-    for d, (n, Ln, seed) in enumerate(shapes):  # ids hashed to 32 bits in even datasets, 64 in odd ones
-        X = cgh.dense_ranks(wl.hashed_ids(wl.zipf_iid(n, Ln, seed), bits=32 if d % 2 == 0 else 64, seed=seed))
+
+    def make(arg):
+        d, (n, Ln, seed) = arg
+        return cgh.dense_ranks(wl.batch_dataset(n, Ln, seed, d))
+
+    with ThreadPoolExecutor(8) as ex:
+        Xs = list(ex.map(make, enumerate(shapes)))
+    sets, taus, tuned, pairs = [], [], [], 0
+    for X in Xs:
+        n = X.shape[0]
         tau = cgh.choose_tau(X, cgh.upper_elems(n, Sb) * 2, cgh.free_vram(0)) or 64
         ds = cgh.Dataset(cgh.Encoded(X, tau, 256), 0)
-        tuned.append(ds.autotune(Sb)[:2])  # kernel and tile of each dataset's own (plain) launch
+        tuned.append(ds.autotune(Sb)[:2])
         sets.append(ds)
         taus.append(tau)
         pairs += n * (n - 1) // 2
-    from collections import Counter
-    log(f"batch workload: {len(shapes)} datasets, {pairs} pairs, generated + encoded + tuned in "
-        f"{time.perf_counter() - t:.1f}s; taus {sorted(set(taus))}; tuned (kernel, tile) {dict(Counter(tuned))}")
+    del Xs
+    log(f"batch: {len(shapes)} datasets, {pairs} pairs, generated + encoded + tuned in "
+        f"{time.perf_counter() - t:.1f}s; taus {sorted(set(taus))}")
     b = cgh.Batch(sets, Sb)
     out = torch.zeros(b.out_elems(), dtype=torch.int16, device="cuda")
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
     base = out.data_ptr()
 
-    def plain():  # each dataset with its tuned kernel and tile
+    def plain():
         for d, ds in enumerate(sets):
             ds.upper(Sb, base + 2 * b.offset(d), sp, 0)
 
     def batched():
         b.run(base, sp, 0, graph=True)
 
-    def timed(fn):
-        for _ in range(args.warmup):
+    def timed(fn, reps):
+        for _ in range(2):
             fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        for _ in range(args.steps):
+        for _ in range(reps):
             fn()
         e1.record(stream)
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / args.steps
+        return e0.elapsed_time(e1) / reps
 
-    plain()
-    torch.cuda.synchronize()
-    ref = out.clone()
-    out.zero_()
-    torch.cuda.synchronize()
-    batched()
-    torch.cuda.synchronize()
-    same = all(bool(torch.equal(out[b.offset(d): b.offset(d) + cgh.upper_elems(ds.n, Sb)],
-                                ref[b.offset(d): b.offset(d) + cgh.upper_elems(ds.n, Sb)]))
-               for d, ds in enumerate(sets))
+    with torch.cuda.stream(stream):
+        plain()
+        torch.cuda.synchronize()
+        ref = out.clone()
+        out.zero_()
+        torch.cuda.synchronize()
+        batched()
+        torch.cuda.synchronize()
+    same = True
+    for d, ds in enumerate(sets):
+        n = ds.n
+        cells = cgh.stripe_offset(n, Sb, (n - 1) // Sb) + (n - ((n - 1) // Sb) * Sb) ** 2
+        seg = slice(b.offset(d), b.offset(d) + cells)
+        same = same and bool(torch.equal(out[seg], ref[seg]))
     del ref
-    ms_plain = timed(plain)
-    ms_batch = timed(batched)
-    log(f"plain {ms_plain:.2f} ms, batched {ms_batch:.2f} ms, identical output: {same}")
-    rec = {
-        "metric": "Gpairs/s exact H, 64 uneven datasets in one batched persistent launch",
-        "value": pairs / (ms_batch * 1e-3) / 1e9, "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_batch, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": PEAKS[2][2],
-        "data": "synthetic (64 seeded Zipf(1.2) long-tail sets, 2% missing, ids hashed to 32/64 bits)",
-        "config": {"workload": "batch64", "datasets": len(shapes), "pairs": pairs, "stripe": Sb,
-                   "n_range": [500, 20000], "L_range": [800, 3000],
-                   "tuned_kernel_tile": {f"{k}/{t}": c for (k, t), c in Counter(tuned).items()}},
-        "plain_per_dataset": {"value": pairs / (ms_plain * 1e-3) / 1e9, "ms_per_step": ms_plain},
-        "speedup_vs_plain": ms_plain / ms_batch, "identical_output": same,
+    reps = max(3, args.steps // 4)
+    ms_plain = timed(plain, reps)
+    ms_batch = timed(batched, reps)
+    log(f"batch: plain {ms_plain:.2f} ms, batched {ms_batch:.2f} ms, identical output: {same}")
+    b.close()
+    for ds in sets:
+        ds.close()
+    return {
+        "workload": "64 datasets, n ~ U[500, 20000], L ~ U[800, 3000], Zipf(1.2), 2% missing, 32/64-bit ids",
+        "pairs": pairs, "stripe": Sb, "steps": reps,
+        "batched": {"ms": ms_batch, "gpairs_s": pairs / (ms_batch * 1e-3) / 1e9,
+                    "form": "mfma_tile_queue per tile size + one rare launch per dataset on 4 streams, HIP graph"},
+        "plain": {"ms": ms_plain, "gpairs_s": pairs / (ms_plain * 1e-3) / 1e9,
+                  "form": "cgh_dataset_upper per dataset (its autotuned kernel and tile), one stream"},
+        "speedup": ms_plain / ms_batch, "identical_output": same,
+        "tuned_kernel_tile": {f"{k}/{t}": c for (k, t), c in Counter(tuned).items()},
     }
-    print(json.dumps(rec), flush=True)
 
 
-def run_serve(args, cgh, wl, torch):
-    """Resident database of 200,000 x 1,142 profiles; requests of 1-256 query
-    isolates (log-uniform) arriving as a Poisson stream at 50 req/s, answered
-    with query x database u16 distances and the 10 nearest rows.  Reports
-    p50/p99 submit-to-result latency, the served Gpairs/s, and the saturated
-    Gpairs/s of the same requests submitted all at once."""
+def serve_record(args, cgh, wl, torch):
+    """A resident 200,000 x 1,142 database (Zipf(1.2) long tail, 2% missing)
+    and requests of 1-256 query profiles (log-uniform sizes).  The
+    continuous-batching server (4 slots, batches of up to 1,024 queries)
+    against plain per-request service (one request at a time: encode, upload,
+    product, rare pass, top-10, download, then the next), each on the same
+    Poisson stream at 50 req/s (latency from the scheduled arrival to the
+    result) and saturated (every request at once).  The 10 nearest rows of
+    every request must be identical in all runs."""
     import threading
     import numpy as np
     n_db, Ls, n_req, rate = 200000, 1142, args.requests, 50.0
@@ -223,64 +342,166 @@ def run_serve(args, cgh, wl, torch):
     calls = wl.zipf_iid(n_db + int(sizes.sum()), Ls, 9, s=1.2, a_min=5, a_max=3000, miss=0.02)
     db_calls, q_calls = calls[:n_db], calls[n_db:]
     del calls
-    # tau by the upper-triangle cost model on the database itself: queries
-    # drawn from the same population meet its rare alleles in the same
-    # proportion (expected partners per query ~ sum c^2 / n)
-    tau = args.tau or cgh.choose_tau(cgh.dense_ranks(db_calls), 0, cgh.free_vram(0)) or 1024
+    tau = args.tau_serve or cgh.choose_tau(cgh.dense_ranks(db_calls), 0, cgh.free_vram(0)) or 1024
     db = cgh.Database(db_calls, tau)
     del db_calls
     log(f"serve: database {n_db}x{Ls} tau={db.tau} one-hot={db.D} resident in {time.perf_counter() - t:.1f}s; "
         f"{n_req} requests, {int(sizes.sum())} queries")
-    sv = cgh.Server(db, slots=4, max_batch=1024)
     bounds = np.concatenate([[0], np.cumsum(sizes)])
     gaps = rng.exponential(1.0 / rate, size=n_req)
+    arrivals = np.cumsum(gaps)
 
-    def run(schedule):
+    def served_run(sv, at):
+        """Requests submitted at their arrival times by a submitter thread,
+        waited for in order; latency from arrival to result."""
         tickets = [None] * n_req
+        t0 = time.perf_counter()
 
         def submitter():
-            t0 = time.perf_counter()
-            at = 0.0
             for k in range(n_req):
-                at += schedule[k]
-                d = t0 + at - time.perf_counter()
+                d = t0 + at[k] - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
                 tickets[k] = sv.submit(q_calls[bounds[k]:bounds[k + 1]], want_dist=False)
 
         th = threading.Thread(target=submitter)
-        t0 = time.perf_counter()
         th.start()
-        lats = []
+        lats, tops = [], []
         for k in range(n_req):
             while tickets[k] is None:
-                time.sleep(0.0005)
-            lats.append(sv.wait(tickets[k])[3])
+                time.sleep(0.0002)
+            _, td, ti, _ = sv.wait(tickets[k])
+            lats.append(time.perf_counter() - (t0 + at[k]))
+            tops.append((td, ti))
         th.join()
-        return time.perf_counter() - t0, np.array(lats)
+        return time.perf_counter() - t0, np.array(lats), tops
 
-    run(np.zeros(n_req))  # warm every slot and kernel once
-    wall_p, lat_p = run(gaps)
-    wall_s, lat_s = run(np.zeros(n_req))
+    def plain_run(sv, at):
+        """One request at a time, each started at its arrival time or when
+        the previous one is done, whichever is later."""
+        t0 = time.perf_counter()
+        lats, tops = [], []
+        for k in range(len(at)):
+            d = t0 + at[k] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            _, td, ti, _ = sv.wait(sv.submit(q_calls[bounds[k]:bounds[k + 1]], want_dist=False))
+            lats.append(time.perf_counter() - (t0 + at[k]))
+            tops.append((td, ti))
+        return time.perf_counter() - t0, np.array(lats), tops
+
+    zeros = np.zeros(n_req)
     pairs = int(sizes.sum()) * n_db
-    p50, p99 = float(np.percentile(lat_p, 50) * 1e3), float(np.percentile(lat_p, 99) * 1e3)
-    log(f"poisson {rate:.0f} req/s: p50 {p50:.1f} ms, p99 {p99:.1f} ms over {wall_p:.1f}s; "
-        f"saturated: {pairs / wall_s / 1e9:.1f} Gpairs/s, p50 {np.percentile(lat_s, 50) * 1e3:.1f} ms")
-    rec = {
-        "metric": "p99 query ms (Poisson 50 req/s, 1-256 queries each, 200k x 1142 resident database)",
-        "value": p99, "unit": "ms", "n_gpus": 1, "steps": n_req, "warmup": n_req,
-        "ms_per_step": wall_p * 1e3 / n_req, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": PEAKS[2][2], "data": "synthetic (seeded Zipf(1.2) long-tail database and queries, 2% missing)",
-        "config": {"workload": "serve", "db_profiles": n_db, "L": Ls, "tau": db.tau, "requests": n_req,
-                   "queries": int(sizes.sum()), "rate_req_s": rate, "slots": sv.slots, "max_batch": 1024},
-        "p50_ms": p50, "p99_ms": p99,
-        "served_gpairs_s": pairs / wall_p / 1e9,
-        "saturated": {"gpairs_s": pairs / wall_s / 1e9, "p50_ms": float(np.percentile(lat_s, 50) * 1e3),
-                      "p99_ms": float(np.percentile(lat_s, 99) * 1e3)},
-    }
-    print(json.dumps(rec), flush=True)
+    sv = cgh.Server(db, slots=4, max_batch=1024)
+    served_run(sv, zeros)  # every slot and kernel warm
+    s_wall, s_lat, s_tops = served_run(sv, arrivals)
+    sat_wall, sat_lat, sat_tops = served_run(sv, zeros)
+    slots = sv.slots
     sv.close()
+    pv = cgh.Server(db, slots=1, max_batch=256)
+    plain_run(pv, zeros[:8])
+    p_wall, p_lat, p_tops = plain_run(pv, arrivals)
+    psat_wall, psat_lat, psat_tops = plain_run(pv, zeros)
+    pv.close()
     db.close()
+    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+               for run in (s_tops, p_tops, psat_tops) for a, b in zip(sat_tops, run))
+    pct = lambda v, q: float(np.percentile(v, q) * 1e3)
+    log(f"serve: served p50 {pct(s_lat, 50):.2f} ms p99 {pct(s_lat, 99):.2f} ms, saturated "
+        f"{pairs / sat_wall / 1e9:.1f} Gpairs/s; plain p50 {pct(p_lat, 50):.2f} ms p99 {pct(p_lat, 99):.2f} ms, "
+        f"saturated {pairs / psat_wall / 1e9:.1f} Gpairs/s; identical top-10: {same}")
+    return {
+        "workload": f"{n_db} x {Ls} resident database, {n_req} requests of 1-256 queries (log-uniform), "
+                    f"{int(sizes.sum())} queries",
+        "tau": int(db.tau), "rate_req_s": rate, "pairs": pairs,
+        "p50_ms": pct(s_lat, 50), "p99_ms": pct(s_lat, 99),
+        "served": {"form": f"continuous batching, {slots} slots, batches of up to 1024 queries",
+                   "poisson": {"p50_ms": pct(s_lat, 50), "p99_ms": pct(s_lat, 99),
+                               "gpairs_s": pairs / s_wall / 1e9},
+                   "saturated": {"gpairs_s": pairs / sat_wall / 1e9, "p50_ms": pct(sat_lat, 50),
+                                 "p99_ms": pct(sat_lat, 99)}},
+        "plain": {"form": "one request at a time (encode, upload, product, rare pass, top-10, download)",
+                  "poisson": {"p50_ms": pct(p_lat, 50), "p99_ms": pct(p_lat, 99), "gpairs_s": pairs / p_wall / 1e9},
+                  "saturated": {"gpairs_s": pairs / psat_wall / 1e9, "p50_ms": pct(psat_lat, 50),
+                                "p99_ms": pct(psat_lat, 99)}},
+        "saturated_speedup": psat_wall / sat_wall,
+        "identical_top10": same,
+    }
+
+
+def native_oracle():
+    """oracle/oracle.c compiled for this host's CPU (-march=native, OpenMP)
+    into a temporary directory, loaded as the oracle's library."""
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    d = tempfile.mkdtemp(prefix="cgh_oracle_native_")
+    so = os.path.join(d, "liboracle_native.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", "-o", so,
+                    os.path.join(ROOT, "oracle", "oracle.c")], check=True)
+    oracle.use_library(so)
+    return oracle
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(X, flat, cgh, wl, target_s=12.0):
+    """The reference's only CPU distance code is the direct compare
+    sum_k [a!=0][b!=0][a!=b] (check.py:39-44, tc_hamming.cu:244-248); the
+    oracle's OpenMP C restatement, built for this CPU, timed on the
+    2,000 x 1,142 set in full and on the first rows of the headline set
+    against all later rows (those rows of the GPU output are checked)."""
+    import numpy as np
+    oracle = native_oracle()
+    n = X.shape[0]
+    # the 2,000 x 1,142 set, whole upper triangle, repeated to at least 1 s
+    X1 = cgh.dense_ranks(wl.small_zipf(2000, 1142, seed=1))
+    reps, t = 0, time.perf_counter()
+    while True:
+        oracle.upper_block(X1, 0, X1.shape[0])
+        reps += 1
+        if time.perf_counter() - t >= 1.0:
+            break
+    c1 = reps * (2000 * 1999 // 2) / (time.perf_counter() - t) / 1e9
+    probe = 256
+    t = time.perf_counter()
+    oracle.upper_block(X, 0, probe)
+    per_row = (time.perf_counter() - t) / probe
+    rows = int(min(n // 2, max(probe, target_s / max(per_row, 1e-9))))
+    t = time.perf_counter()
+    ref = oracle.upper_block(X, 0, rows)
+    dt = time.perf_counter() - t
+    pairs = int(ref.size)
+    parts = []
+    for i in range(rows):
+        s = i // S
+        r0 = s * S
+        off = cgh.stripe_offset(n, S, s) + (i - r0) * (n - r0)
+        parts.append(flat[off + (i + 1 - r0): off + (n - r0)])
+    exact = bool(np.array_equal(np.concatenate(parts), ref))
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {
+        "value": pairs / dt / 1e9, "unit": "Gpairs/s", "cores": oracle.num_threads(), "kind": "port",
+        "sample": f"rows 0..{rows - 1} of the 63,005 x 1,142 set vs all later rows ({pairs} pairs, {dt:.1f} s)",
+        "config1_full": {"gpairs_s": c1, "workload": "2,000 x 1,142 (small_zipf), whole upper triangle",
+                         "repetitions": reps},
+        "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity": affinity,
+        "build": "gcc -O3 -march=native -fopenmp (oracle/oracle.c, built on this host at run time)",
+        "op": "OpenMP C direct compare sum_k [a!=0][b!=0][a!=b] over u16 ranks, 16 x 16 pair tiles",
+        "sample_matches_gpu": exact,
+    }
 
 
 def run_cli(args, cgh, wl):
@@ -335,20 +556,24 @@ def run_cli(args, cgh, wl):
     print(json.dumps(rec), flush=True)
 
 
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--tau", type=int, default=0, help="0 = cost model")
+    ap.add_argument("--tau", type=int, default=0, help="headline tau (0 = cost model)")
+    ap.add_argument("--tau-serve", type=int, default=0, help="serving database tau (0 = cost model)")
     ap.add_argument("--kernel", type=int, default=0,
                     help="CGH_KERNEL_*: 0 auto, 1 int8 MFMA, 2 fp4 MFMA, 3 VALU AND + popcount")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="skip the batch and serve records")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration) and exit")
-    ap.add_argument("--workload", default="long_tail", choices=["long_tail", "batch", "serve", "cli"],
-                    help="long_tail: the 63,005 x 1,142 headline; batch: 64 uneven datasets, batched vs plain; "
-                         "serve: resident 200k database, Poisson query stream")
+    ap.add_argument("--workload", default="all", choices=["all", "long_tail", "batch", "serve", "cli"],
+                    help="all: the headline line with its batch and serve records; long_tail: the headline "
+                         "alone; batch / serve / cli: that record alone")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
     ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="no GPU: turn four rocprofv3 --pmc csv directories into "
@@ -362,7 +587,6 @@ def main():
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     torch.zeros(1, device="cuda")  # torch's HIP runtime first; the library shares it
@@ -377,123 +601,24 @@ def main():
     cgh = load_package()
     import cgmlst_hamming_gpu_amd.workloads as wl
     if args.workload == "batch":
-        return run_batch(args, cgh, wl, torch)
+        print(json.dumps(batch_record(args, cgh, wl, torch)), flush=True)
+        return
     if args.workload == "serve":
-        return run_serve(args, cgh, wl, torch)
+        print(json.dumps(serve_record(args, cgh, wl, torch)), flush=True)
+        return
     if args.workload == "cli":
         return run_cli(args, cgh, wl)
 
-    t = time.perf_counter()
-    calls = wl.long_tail(N, L, seed=2)
-    X = cgh.dense_ranks(calls)
-    del calls
-    log(f"workload long_tail {N}x{L} generated + ranked in {time.perf_counter() - t:.1f}s")
-    tau = args.tau or cgh.choose_tau(X, cgh.upper_elems(N, S) * 2, cgh.free_vram(0), log=log)
-    t = time.perf_counter()
-    enc = cgh.Encoded(X, tau, 256)
-    e = enc.enc
-    D, R = int(e.D), int(e.n_rare)
-    log(f"encoded tau={tau} D={D} rare_pairs={R} kw1={e.kw1} kw2={e.kw2} in {time.perf_counter() - t:.1f}s")
-    ds = cgh.Dataset(enc, 0)
-    elems = cgh.upper_elems(N, S)
-    out = torch.empty(elems, dtype=torch.int16, device="cuda")
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
-
-    def step(ev=None):
-        if ev:
-            ev[0].record(stream)
-        ds.phase(S, out.data_ptr(), cgh.PHASE_PRODUCT, sp, args.kernel)
-        if ev:
-            ev[1].record(stream)
-        ds.phase(S, out.data_ptr(), cgh.PHASE_RARE, sp)
-        if ev:
-            ev[2].record(stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(evs[k])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    t_prod = [evs[k][0].elapsed_time(evs[k][1]) for k in range(args.steps)]
-    t_rare = [evs[k][1].elapsed_time(evs[k][2]) for k in range(args.steps)]
-    ms_step = wall * 1e3 / args.steps
-    pairs = N * (N - 1) // 2
-    prod_ms = sum(t_prod) / len(t_prod)
-    rare_ms = sum(t_rare) / len(t_rare)
-    ops = 2.0 * pairs * (D + L)
-    achieved = ops / (prod_ms * 1e-3) / 1e12
-    kern = 2 if args.kernel == 0 else args.kernel  # the auto choice is fp4
-    peak, punit, dtype = PEAKS[kern]
-    log(f"step {ms_step:.2f} ms (product {prod_ms:.2f} ms, rare {rare_ms:.2f} ms)")
-
-    # HBM bytes per launch of each kernel, from the committed counter passes
-    # (profiles/hbm_traffic_*.json) when they cover this configuration and
-    # were taken on this kernel source
-    traffic = {"product": None, "rare": None}
-    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_long_tail_63005x1142.json")
-    if os.path.exists(tpath):
-        t_rec = json.load(open(tpath))
-        if ((t_rec["n"], t_rec["L"], t_rec["tau"], t_rec["stripe"], t_rec.get("product_kernel")) == (N, L, tau, S, kern)
-                and t_rec.get("kernel_source_sha256") == kernel_source_sha()):
-            traffic = {k: t_rec["kernels"][k]["traffic_bytes"] for k in traffic}
-    # the rare pass: every output cell read and written once (4 B per pair)
-    # plus one member index per rare pair (u16 below 65,536 rows, else u32)
-    rare_bytes = 4.0 * pairs + (2.0 if N <= 65536 else 4.0) * R
-    rare_gbs = rare_bytes / (rare_ms * 1e-3) / 1e9
-    rooflines = {
-        "product": {"bound": "valu" if kern == 3 else "mfma", "achieved": achieved, "peak": peak, "unit": punit,
-                    "frac": achieved / peak, "traffic": traffic["product"],
-                    "kernel": ("valu_tile" if kern == 3 else "mfma_tile") +
-                              " (binary product + u16 epilogue, one launch per step)",
-                    "work": "2*(D+L) ops per pair (multiply-add per bit pair), pairs = n(n-1)/2",
-                    "ms": prod_ms},
-        "rare": {"bound": "hbm", "achieved": rare_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": rare_gbs / HBM_PEAK_GBS, "traffic": traffic["rare"],
-                 "kernel": "sub_rare_rows (rare equal pairs subtracted in LDS, one launch per step)",
-                 "work": "4 B per pair (u16 cell read + write) + 2 B per rare pair (u16 member index)",
-                 "ms": rare_ms},
-    }
-    dominant = "product" if prod_ms >= rare_ms else "rare"
-
-    measured = cgh.measure_peaks(0)
-    log("measured peaks: " + ", ".join(f"{k} {v:.0f}" for k, v in measured.items()))
-    rooflines["product"]["frac_of_measured"] = achieved / measured[MEASURED_PEAK_KEY[kern]]
-    rooflines["rare"]["frac_of_measured"] = rare_gbs / measured["copy_gbs"]
-
-    flat = out.cpu().numpy().view(np.uint16)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(X, flat, cgh)
+    rec, X, flat = headline(args, cgh, wl, torch)
+    if args.workload == "all" and not args.headline_only:
+        rec["batch"] = batch_record(args, cgh, wl, torch)
+        rec["serve"] = serve_record(args, cgh, wl, torch)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(X, flat, cgh, wl)
     if cpu is not None:
-        log(f"cpu baseline {cpu['value']:.4f} Gpairs/s on {cpu['cores']} threads, "
+        log(f"cpu baseline {cpu['value']:.4f} Gpairs/s on {cpu['cores']} threads ({cpu['cpu_model']}), "
+            f"2,000 x 1,142 in full {cpu['config1_full']['gpairs_s']:.4f} Gpairs/s, "
             f"sample matches GPU: {cpu['sample_matches_gpu']}")
-
-    rec = {
-        "metric": "Gpairs/s exact H at 63k x 1142 (u16 upper triangle, device-resident)",
-        "value": pairs / (ms_step * 1e-3) / 1e9,
-        "unit": "Gpairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": dtype,
-        "data": "synthetic (seeded long-tail Zipf(1.1) allele calls, Beta(0.5,15) per-sample missing)",
-        "config": {"workload": "long_tail", "n": N, "L": L, "tau": tau, "stripe": S, "layout": "upper-tri u16",
-                   "one_hot_cols": D, "rare_pairs": R, "tile": ds.tile(), "kernel": kern},
-        "roofline": rooflines[dominant],
-        "rooflines": rooflines,
-        "phases_ms": {"product": prod_ms, "rare": rare_ms},
-        "measured_peaks": measured,
-        "frac_of_measured_peak": achieved / measured[MEASURED_PEAK_KEY[kern]],
-        "cpu_baseline": cpu,
-    }
+    rec["cpu_baseline"] = cpu
     print(json.dumps(rec), flush=True)
 
 

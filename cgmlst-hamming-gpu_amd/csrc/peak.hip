@@ -104,11 +104,18 @@ __global__ void __launch_bounds__(256) stream_read(const v4i* __restrict__ src, 
     const size_t stride = (size_t)gridDim.x * 256;
     int acc = 0;
     for (int p = 0; p < passes; p++) {
-        size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
-        for (; k + (U - 1) * stride < n; k += U * stride) {
+        // each pass reads every word, rotated by half the array on odd
+        // passes, so no pass can reuse another's loads
+        const size_t rot = (p & 1) ? n / 2 : 0;
+        for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += U * stride) {
             v4i v[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) v[u] = src[k + u * stride];
+            for (int u = 0; u < U; u++) {
+                size_t w = k + u * stride;
+                w = w < n ? w + rot : k + rot;  // past the end: reload this lane's first word
+                if (w >= n) w -= n;
+                v[u] = src[w];
+            }
 #pragma unroll
             for (int u = 0; u < U; u++) acc ^= v[u][0] + v[u][1] + v[u][2] + v[u][3];
         }

@@ -44,12 +44,21 @@ LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 _lib = None
 
 
-def lib():
+def use_library(path):
+    """Load the oracle from another build of oracle.c (the CPU baseline uses
+    one compiled for the host it runs on); later calls go to it."""
+    global _lib
+    _lib = None
+    return lib(path)
+
+
+def lib(path=None):
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} not built: make -C oracle")
-        L = C.CDLL(LIB_PATH)
+        path = path or LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built: make -C oracle")
+        L = C.CDLL(path)
         P, I, S = C.c_void_p, C.c_int, C.c_size_t
         L.oracle_hamming_rows.argtypes = [P, I, I, I, I, P]
         L.oracle_hamming_rows.restype = None
