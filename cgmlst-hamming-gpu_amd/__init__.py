@@ -35,6 +35,7 @@ EXPORTS = (
     "cgh_db_create", "cgh_db_info", "cgh_db_free",
     "cgh_server_create", "cgh_server_slots", "cgh_server_submit", "cgh_server_wait", "cgh_server_free",
     "cgh_measure_peaks", "cgh_measure_memory", "cgh_dataset_tile", "cgh_batch_run_noalloc",
+    "cgh_dataset_tuning", "cgh_batch_autotune",
 )
 
 PHASE_PRODUCT, PHASE_RARE = 1, 2
@@ -119,6 +120,8 @@ def load():
         "cgh_dataset_tile": (I, [P, I]),
         "cgh_batch_run_noalloc": (I, [C.POINTER(P), I, I, P, P, I, C.c_char_p, I]),
         "cgh_measure_memory": (I, [I] + [C.POINTER(C.c_double)] * 3 + [C.c_char_p, I]),
+        "cgh_dataset_tuning": (None, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+        "cgh_batch_autotune": (I, [P, P, P, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_double), C.c_char_p, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -313,6 +316,12 @@ class Dataset:
         _check(load().cgh_dataset_autotune(self.handle, S, C.byref(k), C.byref(t), C.byref(ms), err, 512), err)
         return k.value, t.value, ms.value
 
+    def tuning(self):
+        """cgh_dataset_tuning -> (kernel, tile, LDS ring depth) that CGH_KERNEL_AUTO runs."""
+        k, t, st = C.c_int(), C.c_int(), C.c_int()
+        load().cgh_dataset_tuning(self.handle, C.byref(k), C.byref(t), C.byref(st))
+        return k.value, t.value, st.value
+
     def set_cap(self, cap: int):
         """Outputs from now on: values above cap written as cap (0 = none)."""
         if load().cgh_dataset_set_cap(self.handle, int(cap)):
@@ -397,6 +406,14 @@ class Batch:
         err = _err()
         _check(load().cgh_batch_run(self.handle, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), kernel, int(graph), err,
                                     512), err)
+
+    def autotune(self, out_ptr: int, stream_ptr: int = 0):
+        """cgh_batch_autotune -> (row threshold for 256-tiles, kernel, ms of the fastest)."""
+        th, k, ms = C.c_int(), C.c_int(), C.c_double()
+        err = _err()
+        _check(load().cgh_batch_autotune(self.handle, C.c_void_p(out_ptr), C.c_void_p(stream_ptr), C.byref(th),
+                                         C.byref(k), C.byref(ms), err, 512), err)
+        return th.value, k.value, ms.value
 
 
 def batch_run_noalloc(datasets, S: int, out_ptr: int, stream_ptr: int = 0, kernel: int = 0):

@@ -188,18 +188,18 @@ __device__ __forceinline__ Acc mfma_fp4(v8i a, v8i b, Acc c) {
 // Square BM x BM output tile per workgroup, WM x WN per wavefront; FP4
 // selects the fp4 MFMA (64 bits of k per instruction) over the int8 one (32).
 constexpr int KG = KC / 4;   // 4-word groups per slab
-constexpr int NST = 3;       // LDS ring stages (slabs in flight)
+constexpr int NST_DEFAULT = 3;  // LDS ring stages (slabs in flight); the autotuner also tries 2 and 4
 
 // ring[stage][side A/B][group][row] of 4 consecutive words: one
 // wavefront-wide direct-to-LDS load fills 64 consecutive rows of a group
 // (1 KiB, contiguous), and a wavefront's 16-byte reads of 32 consecutive
 // rows are contiguous, hence conflict-free.
-template <int BM>
+template <int BM, int NST>
 using Ring = uint4[NST][2][KG][BM];
 
 // One BM x BM tile (I, J) of stripe s of job a, WM x WN per wavefront.
-template <int BM, int WM, int WN, bool FP4>
-__device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J, Ring<BM>& ring) {
+template <int BM, int WM, int WN, bool FP4, int NST>
+__device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J, Ring<BM, NST>& ring) {
     constexpr int WAVES_N = BM / WN;
     constexpr int NT = (BM / WM) * WAVES_N * 64;
     constexpr int TM = WM / 32, TN = WN / 32;
@@ -450,10 +450,10 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
 // One launch over the closed-form tile numbering of TileArgs.  Workgroups
 // are dealt round-robin to the 8 XCDs; each XCD gets a contiguous run of
 // tiles so concurrently resident tiles share panels in that XCD's L2.
-template <int BM, int WM, int WN, bool FP4>
+template <int BM, int WM, int WN, bool FP4, int NST>
 __global__ void __launch_bounds__((BM / WM) * (BM / WN) * 64)
 mfma_tile(TileArgs a) {
-    __shared__ Ring<BM> ring;
+    __shared__ Ring<BM, NST> ring;
     const long long t = (long long)(blockIdx.x & 7) * a.chunk + (blockIdx.x >> 3);
     if (t >= a.T) return;
     int s = 0;
@@ -467,7 +467,7 @@ mfma_tile(TileArgs a) {
     }
     const long long local = t - (s ? tiles_before(s, a.q, a.nbc) : 0);
     const int tn = a.nbc - s * a.q;
-    tile_body<BM, WM, WN, FP4>(a.job, s, (int)(local / tn), (int)(local % tn), ring);
+    tile_body<BM, WM, WN, FP4, NST>(a.job, s, (int)(local / tn), (int)(local % tn), ring);
 }
 
 // A job read from memory by a uniform index is uniform: say so, so that its
@@ -506,7 +506,7 @@ template <int BM, int WM, int WN, bool FP4>
 __global__ void __launch_bounds__((BM / WM) * (BM / WN) * 64)
 mfma_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* __restrict__ outoff,
                 const int4* __restrict__ tiles, int T, int* __restrict__ next) {
-    __shared__ Ring<BM> ring;
+    __shared__ Ring<BM, NST_DEFAULT> ring;
     __shared__ int slot;
     for (;;) {
         if (threadIdx.x == 0) slot = atomicAdd(next, 1);
@@ -518,7 +518,7 @@ mfma_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* 
         TileJob jb = jobs[tl.x];
         jb.out = base + outoff[tl.x];
         scalarize(jb);
-        tile_body<BM, WM, WN, FP4>(jb, tl.y, tl.z, tl.w, ring);
+        tile_body<BM, WM, WN, FP4, NST_DEFAULT>(jb, tl.y, tl.z, tl.w, ring);
     }
 }
 
@@ -707,6 +707,17 @@ valu_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* 
 }
 
 __global__ void reset_counter(int* c) { *c = 0; }
+
+// Counts the 16-byte words in which a and b differ (into *diff, cleared by
+// the caller): the autotuner's check that every candidate agrees.
+__global__ void count_diff(const uint4* __restrict__ a, const uint4* __restrict__ b, size_t words, int* diff) {
+    int bad = 0;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < words; k += (size_t)gridDim.x * blockDim.x) {
+        const uint4 x = a[k], y = b[k];
+        bad |= (x.x != y.x) | (x.y != y.y) | (x.z != y.z) | (x.w != y.w);
+    }
+    if (bad) atomicAdd(diff, 1);
+}
 
 // Rare pairs, row-local: a workgroup takes items (row i and a chunk of RCW
 // columns, or a group of short rows) one after the other.  An item's cells
@@ -1209,14 +1220,41 @@ hipError_t upload_members(const uint32_t* mem, size_t count, bool m16, void** ou
     return e;
 }
 
+// One product launch over the tiles of `a` with the kernel form, tile edge and
+// LDS ring depth (2, 3 or 4 slabs) given; grid = a.chunk * 8 workgroups.
+hipError_t launch_tile_args(const TileArgs& a, int kernel, int tile, int nst, hipStream_t st) {
+    const dim3 grid((unsigned)(a.chunk * 8));
+    const bool fp4 = kernel == CGH_KERNEL_MFMA_FP4;
+    if (kernel == CGH_KERNEL_VALU_POPC) {
+        if (tile == 256) hipLaunchKernelGGL((valu_tile<256>), grid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((valu_tile<128>), grid, dim3(256), 0, st, a);
+        return hipGetLastError();
+    }
+#define CGH_MFMA_LAUNCH(BM, WM, NT, F, K)                                                            \
+    hipLaunchKernelGGL((mfma_tile<BM, WM, 64, F, K>), grid, dim3(NT), 0, st, a)
+#define CGH_MFMA_STAGES(BM, WM, NT, F)           \
+    switch (nst) {                               \
+        case 2: CGH_MFMA_LAUNCH(BM, WM, NT, F, 2); break; \
+        case 4: CGH_MFMA_LAUNCH(BM, WM, NT, F, 4); break; \
+        default: CGH_MFMA_LAUNCH(BM, WM, NT, F, 3); break; \
+    }
+    if (tile == 256 && fp4) { CGH_MFMA_STAGES(256, 128, 512, true) }
+    else if (tile == 256) { CGH_MFMA_STAGES(256, 128, 512, false) }
+    else if (fp4) { CGH_MFMA_STAGES(128, 64, 256, true) }
+    else { CGH_MFMA_STAGES(128, 64, 256, false) }
+#undef CGH_MFMA_STAGES
+#undef CGH_MFMA_LAUNCH
+    return hipGetLastError();
+}
+
 hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int ld, int n,
                         int r0, int S, int ns, uint16_t* out, hipStream_t st, const uint32_t* rstart = NULL,
-                        int cap = 0, int kv1 = 0, int kv2 = 0) {
+                        int cap = 0, int kv1 = 0, int kv2 = 0, int nst = NST_DEFAULT) {
     if (ns > 1 && S % tile) {
         for (int s = 0; s < ns; s++) {
             const hipError_t e = launch_tiles(kernel, tile, O, kw1, P, kw2, ld, n, r0 + s * S, S, 1,
                                               out + stripe_off((size_t)(n - r0), (size_t)S, (size_t)s), st, rstart,
-                                              cap, kv1, kv2);
+                                              cap, kv1, kv2, nst);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -1235,20 +1273,7 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
     const long long tm_last = (std::min(S, n - last) + tile - 1) / tile;
     a.T = (ns > 1 ? tiles_before(ns - 1, a.q, a.nbc) : 0) + tm_last * (a.nbc - (long long)(ns - 1) * a.q);
     a.chunk = (a.T + 7) / 8;
-    const dim3 grid((unsigned)(a.chunk * 8));
-    const bool fp4 = kernel == CGH_KERNEL_MFMA_FP4;
-    if (kernel == CGH_KERNEL_VALU_POPC) {
-        if (tile == 256) hipLaunchKernelGGL((valu_tile<256>), grid, dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((valu_tile<128>), grid, dim3(256), 0, st, a);
-    } else if (tile == 256 && fp4)
-        hipLaunchKernelGGL((mfma_tile<256, 128, 64, true>), grid, dim3(512), 0, st, a);
-    else if (tile == 256)
-        hipLaunchKernelGGL((mfma_tile<256, 128, 64, false>), grid, dim3(512), 0, st, a);
-    else if (fp4)
-        hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, st, a);
-    return hipGetLastError();
+    return launch_tile_args(a, kernel, tile, nst, st);
 }
 
 // CGH_KERNEL_AUTO resolves to the fastest measured product kernel.
@@ -1286,7 +1311,7 @@ struct cgh_dataset {
     int n, L, tau;
     size_t D;                 // one-hot columns
     int cap;                  // > 0: outputs capped (cgh_dataset_set_cap)
-    int tuned_kernel, tuned_tile;  // from cgh_dataset_autotune (0 = not tuned)
+    int tuned_kernel, tuned_tile, tuned_stages;  // from cgh_dataset_autotune (0 = not tuned)
     size_t n_pad, kw1, kw2;   // kw1, kw2: words per row on the device (multiples of KC)
     uint32_t *dO, *dP;        // planes of n_pad rows
     size_t n_groups, n_members;
@@ -1503,67 +1528,104 @@ int ds_kv2(const cgh_dataset* ds) { return (int)(((size_t)ds->L + 31) / 32); }
 int ds_tile(const cgh_dataset* ds, int kernel) {
     return ds->tuned_tile && ds_kernel(ds, kernel) == ds->tuned_kernel ? ds->tuned_tile : pick_tile(ds->n);
 }
+// LDS ring depth of the dataset's product launches: the tuned one with the
+// tuned kernel, else the default.
+int ds_stages(const cgh_dataset* ds, int kernel) {
+    return ds->tuned_stages && ds_kernel(ds, kernel) == ds->tuned_kernel ? ds->tuned_stages : NST_DEFAULT;
+}
 
 }  // namespace
 
 extern "C" int cgh_dataset_tile(const cgh_dataset* ds, int kernel) { return ds_tile(ds, kernel); }
 
+extern "C" void cgh_dataset_tuning(const cgh_dataset* ds, int* kernel, int* tile, int* stages) {
+    if (kernel) *kernel = ds_kernel(ds, CGH_KERNEL_AUTO);
+    if (tile) *tile = ds_tile(ds, CGH_KERNEL_AUTO);
+    if (stages) *stages = ds_stages(ds, CGH_KERNEL_AUTO);
+}
+
 extern "C" int cgh_dataset_autotune(cgh_dataset* ds, int S, int* kernel_out, int* tile_out, double* ms_out,
                                     char* err, int errlen) {
-    // Candidates: fp4 and int8 MFMA x 128- and 256-tiles (the VALU form is
-    // never faster), each timed on the first stripe (S rows, the widest)
-    // after one warm-up launch; the fastest is kept.
+    // Candidates: fp4 and int8 MFMA x 128- and 256-tiles x LDS rings of 2,
+    // 3 and 4 slabs (the VALU form is never faster), each timed on the first
+    // stripe (S rows, the widest) after one warm-up launch; the fastest is
+    // kept.  Every candidate's stripe is compared with the first one's: a
+    // candidate that disagrees is an error, not a choice.
     const int n = ds->n;
     const int ns = 1;
-    const int kernels[2] = {CGH_KERNEL_MFMA_FP4, CGH_KERNEL_MFMA_I8}, tiles[2] = {256, 128};
-    uint16_t* buf = NULL;
+    const int kernels[2] = {CGH_KERNEL_MFMA_FP4, CGH_KERNEL_MFMA_I8}, tiles[2] = {256, 128}, stages[3] = {2, 3, 4};
+    const size_t bytes = up(stripe_off((size_t)n, (size_t)S, (size_t)ns), 8) * 2 + 16;
+    uint16_t *buf = NULL, *first = NULL;
+    int* ddiff = NULL;
     hipStream_t st = NULL;
     hipEvent_t e0 = NULL, e1 = NULL;
     double best = 1e300;
-    int bk = 0, bt = 0;
+    int bk = 0, bt = 0, bs = 0, cand = 0;
     if (S < 8 || S % 8) {
         set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
         return 1;
     }
     HIP_TRY(hipSetDevice(ds->device));
-    HIP_TRY(hipMalloc(&buf, up(stripe_off((size_t)n, (size_t)S, (size_t)ns), 8) * 2 + 16));
+    HIP_TRY(hipMalloc(&buf, bytes));
+    HIP_TRY(hipMalloc(&first, bytes));
+    HIP_TRY(hipMalloc(&ddiff, sizeof(int)));
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_TRY(hipMemsetAsync(buf, 0, bytes, st));
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     for (int k : kernels)
-        for (int t : tiles) {
-            float ms = 0.f;
-            HIP_TRY(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf, st,
-                                 NULL, 0, ds_kv1(ds), ds_kv2(ds)));
-            HIP_TRY(hipEventRecord(e0, st));
-            for (int rep = 0; rep < 3; rep++)
+        for (int t : tiles)
+            for (int g : stages) {
+                float ms = 0.f;
+                int diff = 0;
                 HIP_TRY(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf,
-                                     st, NULL, 0, ds_kv1(ds), ds_kv2(ds)));
-            HIP_TRY(hipEventRecord(e1, st));
-            HIP_TRY(hipEventSynchronize(e1));
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            if (ms / 3 < best) {
-                best = ms / 3;
-                bk = k;
-                bt = t;
+                                     st, NULL, 0, ds_kv1(ds), ds_kv2(ds), g));
+                HIP_TRY(hipEventRecord(e0, st));
+                for (int rep = 0; rep < 3; rep++)
+                    HIP_TRY(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns,
+                                         buf, st, NULL, 0, ds_kv1(ds), ds_kv2(ds), g));
+                HIP_TRY(hipEventRecord(e1, st));
+                if (cand++ == 0) {
+                    HIP_TRY(hipMemcpyAsync(first, buf, bytes, hipMemcpyDeviceToDevice, st));
+                } else {
+                    HIP_TRY(hipMemsetAsync(ddiff, 0, sizeof(int), st));
+                    hipLaunchKernelGGL(count_diff, dim3(1024), dim3(256), 0, st, (const uint4*)buf,
+                                       (const uint4*)first, bytes / 16, ddiff);
+                    HIP_TRY(hipGetLastError());
+                    HIP_TRY(hipMemcpyAsync(&diff, ddiff, sizeof(int), hipMemcpyDeviceToHost, st));
+                }
+                HIP_TRY(hipStreamSynchronize(st));
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                if (diff) {
+                    set_err(err, errlen, "autotune: kernel %d, tile %d, ring %d disagrees with kernel %d, tile %d, ring %d",
+                            k, t, g, kernels[0], tiles[0], stages[0]);
+                    goto out;
+                }
+                if (ms / 3 < best) {
+                    best = ms / 3;
+                    bk = k;
+                    bt = t;
+                    bs = g;
+                }
             }
-        }
     ds->tuned_kernel = bk;
     ds->tuned_tile = bt;
+    ds->tuned_stages = bs;
     if (kernel_out) *kernel_out = bk;
     if (tile_out) *tile_out = bt;
     if (ms_out) *ms_out = best;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(st);
-    (void)hipFree(buf);
-    return 0;
+    cand = 0;
+    goto out;
 fail:
+    cand = -1;
+out:
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);
+    if (ddiff) (void)hipFree(ddiff);
+    if (first) (void)hipFree(first);
     if (buf) (void)hipFree(buf);
-    return 1;
+    return cand == 0 ? 0 : 1;
 }
 
 extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, int phases,
@@ -1585,7 +1647,8 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
     HIP_TRY(hipSetDevice(ds->device));
     if (phases & CGH_PHASE_PRODUCT)
         HIP_TRY(launch_tiles(ds_kernel(ds, kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
-                             S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds)));
+                             S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds),
+                             ds_stages(ds, kernel)));
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         RareJob jb = rare_job(d_out, n, 0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap,
                               lanes_per_range(ds->n_rare, ds->n_rentries));
@@ -1609,7 +1672,8 @@ extern "C" int cgh_dataset_upper(cgh_dataset* ds, int S, uint16_t* d_out, void* 
 namespace {
 // Tile edge of a dataset inside a batch: 256 from 4,096 rows (one persistent
 // queue keeps the chip busy with fewer, larger tiles), else 128.
-int batch_tile(int n) { return n >= 4096 ? 256 : 128; }
+constexpr int BATCH_TILE_ROWS = 4096;
+int batch_tile(int n) { return n >= BATCH_TILE_ROWS ? 256 : 128; }
 }  // namespace
 
 struct cgh_batch {
@@ -1618,6 +1682,11 @@ struct cgh_batch {
     size_t total;                       // u16 elements of all outputs
     std::vector<size_t> offs;           // output offset of each dataset (16-byte aligned)
     std::vector<int> tile;              // tile edge chosen per dataset
+    std::vector<const cgh_dataset*> sets;  // borrowed
+    int fixed_tile = 0;                 // cgh_batch_create's tile (0 = by the threshold)
+    int threshold = BATCH_TILE_ROWS;    // 256-tiles for datasets from this many rows
+    int tuned_kernel = 0;               // cgh_batch_autotune's kernel for CGH_KERNEL_AUTO (0 = fp4)
+    size_t tile_cap[2] = {0, 0};        // capacity of the device work lists
     TileJob* djobs = NULL;              // per dataset (out filled in-kernel)
     size_t* doffs = NULL;
     int4* dtiles[2] = {NULL, NULL};     // work lists of the 256 and 128 tile kernels
@@ -1655,16 +1724,65 @@ extern "C" void cgh_batch_free(cgh_batch* b) {
     delete b;
 }
 
+namespace {
+
+// The two device work lists of (dataset, stripe, I, J) tiles for the current
+// tile rule, longest first (below-diagonal zero tiles last); a captured graph
+// refers to the old lists, so it is dropped.
+hipError_t build_queues(cgh_batch* b) {
+    std::vector<std::pair<long long, int4>> lists[2];  // (cost, tile) per kernel
+    for (int d = 0; d < b->count; d++) {
+        const cgh_dataset* ds = b->sets[(size_t)d];
+        const int tl = b->fixed_tile ? b->fixed_tile : (ds->n >= b->threshold ? 256 : 128);
+        b->tile[(size_t)d] = tl;
+        const int k = tl == 256 ? 0 : 1;
+        const long long cost = (long long)(ds->kw1 + ds->kw2);
+        for (int s = 0, r0 = 0; r0 < ds->n; s++, r0 += b->S) {
+            const int rows = std::min(b->S, ds->n - r0), w = ds->n - r0;
+            for (int I = 0; I * tl < rows; I++)
+                for (int J = 0; J * tl < w; J++) lists[k].push_back({J >= I ? cost : 0, make_int4(d, s, I, J)});
+        }
+    }
+    if (b->exec) {
+        const hipError_t e = hipGraphExecDestroy(b->exec);
+        b->exec = NULL;
+        if (e != hipSuccess) return e;
+    }
+    for (int k = 0; k < 2; k++) {
+        std::stable_sort(lists[k].begin(), lists[k].end(),
+                         [](const std::pair<long long, int4>& x, const std::pair<long long, int4>& y) {
+                             return x.first > y.first;
+                         });
+        std::vector<int4> t;
+        t.reserve(lists[k].size());
+        for (auto& e : lists[k]) t.push_back(e.second);
+        b->ntiles[k] = (int)t.size();
+        if (t.empty()) continue;
+        hipError_t e = hipSuccess;
+        if (t.size() > b->tile_cap[k]) {
+            if (b->dtiles[k]) e = hipFree(b->dtiles[k]);
+            b->dtiles[k] = NULL;
+            if (!e) e = hipMalloc(&b->dtiles[k], sizeof(int4) * t.size());
+            if (e) return e;
+            b->tile_cap[k] = t.size();
+        }
+        if ((e = hipMemcpy(b->dtiles[k], t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice))) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
 extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count, int S, int tile, cgh_batch** out,
                                 char* err, int errlen) {
     cgh_batch* b = new cgh_batch();
     std::vector<TileJob> jobs((size_t)count);
-    std::vector<std::pair<long long, int4>> lists[2];  // (cost, tile) per kernel
     *out = NULL;
     b->device = device;
     b->count = count;
     b->S = S;
     b->total = 0;
+    b->fixed_tile = tile;
     if (count < 1 || S < 8 || S % 8 || (tile != 0 && tile != 128 && tile != 256)) {
         set_err(err, errlen, "cgh_batch_create: bad arguments (count=%d S=%d tile=%d)", count, S, tile);
         delete b;
@@ -1677,23 +1795,12 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
             delete b;
             return 1;
         }
-        // per-dataset tile: 256 when its triangle holds enough 256-tiles to
-        // fill the chip, else 128
-        // (a dataset's own tuned tile is not used: inside one persistent
-        // launch 256-tiles pay off from smaller n than in a launch of its own)
-        const int tl = tile ? tile : batch_tile(ds->n);
-        b->tile.push_back(tl);
+        b->sets.push_back(ds);
+        b->tile.push_back(0);
         b->offs.push_back(b->total);
         b->total += cgh_upper_elems(ds->n, S);
         jobs[d] = {ds->dO, ds->dP, (int)ds->kw1, (int)ds->kw2, ds->n, 0, S, NULL, ds->dO, ds->dP, ds->n, 0,
                    (int)ds->n_pad, (int)ds->n_pad, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds)};
-        const int k = tl == 256 ? 0 : 1;
-        const long long cost = (long long)(ds->kw1 + ds->kw2);
-        for (int s = 0, r0 = 0; r0 < ds->n; s++, r0 += S) {
-            const int rows = std::min(S, ds->n - r0), w = ds->n - r0;
-            for (int I = 0; I * tl < rows; I++)
-                for (int J = 0; J * tl < w; J++) lists[k].push_back({J >= I ? cost : 0, make_int4(d, s, I, J)});
-        }
         if (ds->n_rare && ds->n > 1) {
             RareJob r = rare_job(NULL, ds->n, 0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap,
                                  lanes_per_range(ds->n_rare, ds->n_rentries));
@@ -1708,19 +1815,7 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
     HIP_TRY(hipMemcpy(b->djobs, jobs.data(), sizeof(TileJob) * (size_t)count, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&b->doffs, sizeof(size_t) * (size_t)count));
     HIP_TRY(hipMemcpy(b->doffs, b->offs.data(), sizeof(size_t) * (size_t)count, hipMemcpyHostToDevice));
-    for (int k = 0; k < 2; k++) {
-        // longest tiles first; below-diagonal (zero-fill) tiles last
-        std::stable_sort(lists[k].begin(), lists[k].end(),
-                         [](const std::pair<long long, int4>& x, const std::pair<long long, int4>& y) {
-                             return x.first > y.first;
-                         });
-        std::vector<int4> t;
-        for (auto& e : lists[k]) t.push_back(e.second);
-        b->ntiles[k] = (int)t.size();
-        if (t.empty()) continue;
-        HIP_TRY(hipMalloc(&b->dtiles[k], sizeof(int4) * t.size()));
-        HIP_TRY(hipMemcpy(b->dtiles[k], t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice));
-    }
+    HIP_TRY(build_queues(b));
     HIP_TRY(hipMalloc(&b->dcount, 2 * sizeof(int)));
     if (!b->rare.empty()) {  // largest first, each to the least loaded stream
         std::vector<size_t> order(b->rare.size());
@@ -1757,6 +1852,7 @@ namespace {
 // The whole batch: counter resets, the persistent product kernel per tile
 // size, then one rare-pair launch over every dataset's rows.
 hipError_t batch_enqueue(cgh_batch* b, uint16_t* d_out, hipStream_t st, int kernel) {
+    if (kernel == CGH_KERNEL_AUTO && b->tuned_kernel) kernel = b->tuned_kernel;
     const bool fp4 = resolve_kernel(kernel) == CGH_KERNEL_MFMA_FP4;
     const unsigned cus = (unsigned)b->cus;
     for (int k = 0; k < 2; k++) hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, b->dcount + k);
@@ -1847,6 +1943,91 @@ extern "C" int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int ke
 fail:
     if (graph) (void)hipGraphDestroy(graph);
     return 1;
+}
+
+// Batch autotuner: the whole batch timed (eagerly, on `stream`, into d_out)
+// under each candidate tile rule (256-tiles for every dataset, from 2,048,
+// 4,096, 8,192 or 16,384 rows, or none) and product kernel (fp4, int8 MFMA);
+// the fastest rule and kernel are kept for later runs with CGH_KERNEL_AUTO.
+// A batch's best tile per dataset depends on what else shares the queue, so
+// the batch is tuned as a whole rather than from each dataset's own timing.
+extern "C" int cgh_batch_autotune(cgh_batch* b, uint16_t* d_out, void* stream, int* threshold_out, int* kernel_out,
+                                  double* ms_out, char* err, int errlen) {
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    const int thresholds[6] = {0, 2048, 4096, 8192, 16384, INT32_MAX};
+    const int kernels[2] = {CGH_KERNEL_MFMA_FP4, CGH_KERNEL_MFMA_I8};
+    const size_t bytes = b->total * 2;
+    uint16_t* first = NULL;
+    int* ddiff = NULL;
+    double best = 1e300;
+    int bth = b->threshold, bk = CGH_KERNEL_MFMA_FP4, cand = 0;
+    const int fixed = b->fixed_tile, threshold = b->threshold;
+    if (((uintptr_t)d_out) & 15) {
+        set_err(err, errlen, "output buffer must be 16-byte aligned");
+        return 1;
+    }
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(st));  // earlier runs may still read the work lists
+    HIP_TRY(hipMalloc(&first, bytes));
+    HIP_TRY(hipMalloc(&ddiff, sizeof(int)));
+    HIP_TRY(hipMemsetAsync(d_out, 0, bytes, st));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    b->fixed_tile = 0;
+    for (int th : thresholds) {
+        b->threshold = th;
+        HIP_TRY(build_queues(b));
+        for (int k : kernels) {
+            float ms = 0.f;
+            int diff = 0;
+            HIP_TRY(batch_enqueue(b, d_out, st, k));
+            HIP_TRY(hipEventRecord(e0, st));
+            for (int rep = 0; rep < 2; rep++) HIP_TRY(batch_enqueue(b, d_out, st, k));
+            HIP_TRY(hipEventRecord(e1, st));
+            if (cand++ == 0) {  // every candidate must write what the first one wrote
+                HIP_TRY(hipMemcpyAsync(first, d_out, bytes, hipMemcpyDeviceToDevice, st));
+            } else {
+                HIP_TRY(hipMemsetAsync(ddiff, 0, sizeof(int), st));
+                hipLaunchKernelGGL(count_diff, dim3(1024), dim3(256), 0, st, (const uint4*)d_out, (const uint4*)first,
+                                   bytes / 16, ddiff);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipMemcpyAsync(&diff, ddiff, sizeof(int), hipMemcpyDeviceToHost, st));
+            }
+            HIP_TRY(hipStreamSynchronize(st));
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (diff) {
+                set_err(err, errlen, "autotune: threshold %d, kernel %d disagrees with threshold %d, kernel %d", th, k,
+                        thresholds[0], kernels[0]);
+                b->threshold = threshold;
+                b->fixed_tile = fixed;
+                (void)build_queues(b);
+                goto out;
+            }
+            if (ms / 2 < best) {
+                best = ms / 2;
+                bth = th;
+                bk = k;
+            }
+        }
+    }
+    b->threshold = bth;
+    b->tuned_kernel = bk;
+    HIP_TRY(build_queues(b));
+    if (threshold_out) *threshold_out = bth;
+    if (kernel_out) *kernel_out = bk;
+    if (ms_out) *ms_out = best;
+    cand = 0;
+    goto out;
+fail:
+    b->fixed_tile = fixed;
+    cand = -1;
+out:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (ddiff) (void)hipFree(ddiff);
+    if (first) (void)hipFree(first);
+    return cand == 0 ? 0 : 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -2314,16 +2495,7 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     a.nbc = (db->n + tile - 1) / tile;
     a.T = (long long)((q + tile - 1) / tile) * a.nbc;
     a.chunk = (a.T + 7) / 8;
-    const bool fp4 = resolve_kernel(sv->kernel) == CGH_KERNEL_MFMA_FP4;
-    const dim3 grid((unsigned)(a.chunk * 8));
-    if (sv->kernel == CGH_KERNEL_VALU_POPC && tile == 256)
-        hipLaunchKernelGGL((valu_tile<256>), grid, dim3(256), 0, sl.st, a);
-    else if (sv->kernel == CGH_KERNEL_VALU_POPC)
-        hipLaunchKernelGGL((valu_tile<128>), grid, dim3(256), 0, sl.st, a);
-    else if (tile == 256 && fp4) hipLaunchKernelGGL((mfma_tile<256, 128, 64, true>), grid, dim3(512), 0, sl.st, a);
-    else if (tile == 256) hipLaunchKernelGGL((mfma_tile<256, 128, 64, false>), grid, dim3(512), 0, sl.st, a);
-    else if (fp4) hipLaunchKernelGGL((mfma_tile<128, 64, 64, true>), grid, dim3(256), 0, sl.st, a);
-    else hipLaunchKernelGGL((mfma_tile<128, 64, 64, false>), grid, dim3(256), 0, sl.st, a);
+    if ((e = launch_tile_args(a, resolve_kernel(sv->kernel), tile, NST_DEFAULT, sl.st))) return e;
     if (ne) {
         RareJob jb = rare_job(sl.ddist, q, 0, q, sl.drs, sl.dre, db->dqmem, db->m16, 0, 32);
         jb.rect = 1;
@@ -2813,6 +2985,7 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
     if (!stripe_height_ok(S, err, errlen)) return 1;
     const auto t_start = std::chrono::steady_clock::now();
     const int n = ds->n, kernel = ds_kernel(ds, CGH_KERNEL_AUTO), tile = ds_tile(ds, CGH_KERNEL_AUTO);
+    const int stages = ds_stages(ds, CGH_KERNEL_AUTO);
     const int rsg = lanes_per_range(ds->n_rare, ds->n_rentries);
     StripeRing ring;
     HIP_TRY(hipSetDevice(ds->device));
@@ -2820,7 +2993,7 @@ extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, 
     {
         auto produce = [&](int r0, int rows, uint16_t* dst, hipStream_t st) -> hipError_t {
             hipError_t e = launch_tiles(kernel, tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n,
-                                        r0, S, 1, dst, st, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds));
+                                        r0, S, 1, dst, st, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds), stages);
             if (!e && ds->n_rare && n - r0 > 1) {
                 RareJob jb = rare_job(dst, n, r0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap, rsg);
                 e = launch_rare(jb, rare_layout(jb, rows), st);

@@ -172,9 +172,11 @@ int cgh_dataset_encode(int device, const uint64_t* calls, int n, int L, int tau,
 int cgh_dataset_set_cap(cgh_dataset* ds, int cap);
 
 /* Autotuner: time the product kernel candidates (fp4 and int8 MFMA, 128- and
- * 256-tiles) on the dataset's first stripe of S rows and keep the fastest
- * for CGH_KERNEL_AUTO in every later call on this dataset (upper, phase,
- * stream; a batch keeps its own tile rule).  Outputs
+ * 256-tiles, LDS rings of 2, 3 and 4 slabs: 12 configurations) on the
+ * dataset's first stripe of S rows and keep the fastest for CGH_KERNEL_AUTO
+ * in every later call on this dataset (upper, phase, stream; a batch is
+ * tuned as a whole by cgh_batch_autotune).  The stripe height is not a
+ * candidate: it fixes the caller's output layout.  Outputs
  * (may be NULL): the kernel (CGH_KERNEL_*), the tile edge and its time in
  * ms.  No counterpart in the reference (its CUTLASS tile is fixed at build
  * time, tensorcore/README.md:31-32). */
@@ -253,6 +255,17 @@ int cgh_batch_tile(const cgh_batch* b, int d);
  * must not overlap (they share its side streams and events). */
 int cgh_batch_run(cgh_batch* b, uint16_t* d_out, void* stream, int kernel, int use_graph, char* err, int errlen);
 void cgh_batch_free(cgh_batch* b);
+
+/* Autotuner of a whole batch: times it (eager runs into d_out on `stream`)
+ * under each tile rule (256-tiles for every dataset, for datasets from
+ * 2,048 / 4,096 / 8,192 / 16,384 rows, or none) and each MFMA product form
+ * (fp4, int8), and keeps the fastest for later cgh_batch_run calls with
+ * CGH_KERNEL_AUTO (a captured graph is re-captured).  Outputs (may be NULL):
+ * the row threshold for 256-tiles (0 = all, INT32_MAX = none), the kernel
+ * and its time in ms.  No counterpart in the reference (its tile shape is a
+ * compile-time constant, tensorcore/README.md:31-32). */
+int cgh_batch_autotune(cgh_batch* b, uint16_t* d_out, void* stream, int* threshold, int* kernel, double* ms,
+                       char* err, int errlen);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -358,6 +371,11 @@ int cgh_measure_memory(int device, double* copy_gbs, double* read_gbs, double* c
 
 /* Tile shape (rows = cols) used by cgh_dataset_upper for this dataset. */
 int cgh_dataset_tile(const cgh_dataset* ds, int kernel);
+
+/* The product configuration CGH_KERNEL_AUTO runs on this dataset: kernel
+ * (CGH_KERNEL_*), tile edge and LDS ring depth in 512-bit slabs (2, 3 or 4;
+ * cgh_dataset_autotune chooses among them).  Any pointer may be NULL. */
+void cgh_dataset_tuning(const cgh_dataset* ds, int* kernel, int* tile, int* stages);
 
 #ifdef __cplusplus
 }

@@ -144,9 +144,11 @@ def test_autotune_then_exact(cgh, gpu, workloads, shape):
     X = cgh.dense_ranks(workloads.small_zipf(n, L, seed=11))
     ds = cgh.Dataset(cgh.Encoded(X, 32, 256))
     print(f"autotune n={n} S={S}", flush=True)
-    k, t, ms = ds.autotune(S)
+    k, t, ms = ds.autotune(S)  # raises if any of the 12 candidates disagrees with the first
     assert k in (1, 2) and t in (128, 256) and ms > 0
     assert ds.tile(0) == t
+    tk, tt, stages = ds.tuning()
+    assert (tk, tt) == (k, t) and stages in (2, 3, 4)
     out = gpu.zeros(cgh.upper_elems(n, S), dtype=gpu.int16, device="cuda")
     ds.upper(S, out.data_ptr(), gpu.cuda.current_stream().cuda_stream, 0)
     gpu.cuda.synchronize()
@@ -249,6 +251,43 @@ def test_batch_matches_oracle_and_graph_replays(cgh, gpu, workloads, kernel):
         gpu.cuda.synchronize()
         got = squares(out.cpu().numpy().view(np.uint16))
         assert all(np.array_equal(x, y) for x, y in zip(got, eager)), rep
+    b.close()
+    for s in sets:
+        s.close()
+
+
+def test_batch_autotune_then_exact(cgh, gpu, workloads):
+    """The batch autotuner times every tile rule and product kernel over the
+    whole batch (each candidate's output checked against the first's inside
+    the call); the batch then runs the chosen rule under CGH_KERNEL_AUTO,
+    eager and from a graph, every dataset equal to the oracle."""
+    shapes = [(300, 97, 21), (2100, 300, 22), (4500, 200, 23), (9000, 120, 24)]
+    S = 1024
+    sets, Xs = [], []
+    for n, L, seed in shapes:
+        X = cgh.dense_ranks(workloads.zipf_iid(n, L, seed))
+        Xs.append(X)
+        sets.append(cgh.Dataset(cgh.Encoded(X, 16, 256)))
+    b = cgh.Batch(sets, S)
+    out = gpu.zeros((b.out_elems(),), dtype=gpu.int16, device="cuda")
+    st = gpu.cuda.current_stream()
+    print("batch autotune", flush=True)
+    th, k, ms = b.autotune(out.data_ptr(), st.cuda_stream)
+    assert th in (0, 2048, 4096, 8192, 16384, 2**31 - 1) and k in (1, 2) and ms > 0
+    assert [b.tile(d) for d in range(len(shapes))] == [256 if n >= th else 128 for n, _, _ in shapes]
+    want = [oracle.brute_full(X) for X in Xs]
+    side = gpu.cuda.Stream()
+    for graph, stream in ((False, st), (True, side), (True, side)):
+        out.fill_(0x2b2b)
+        gpu.cuda.synchronize()
+        print(f"batch tuned graph={graph}", flush=True)
+        b.run(out.data_ptr(), stream.cuda_stream, 0, graph=graph)
+        gpu.cuda.synchronize()
+        flat = out.cpu().numpy().view(np.uint16)
+        for d, X in enumerate(Xs):
+            n = X.shape[0]
+            got = cgh.upper_to_square(flat[b.offset(d): b.offset(d) + cgh.upper_elems(n, S)], n, S)
+            assert np.array_equal(got, want[d]), (graph, d)
     b.close()
     for s in sets:
         s.close()
