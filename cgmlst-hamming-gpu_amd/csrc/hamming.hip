@@ -103,6 +103,7 @@ struct TileArgs {
     TileJob job;
     int ns;             // stripes in the launch
     int q;              // row tiles of a full stripe (ns > 1)
+    int qc;             // column tiles the stripe height spans (S / tile width)
     int nbc;            // column tiles of the first stripe
     long long T;        // tiles in the launch
     long long chunk;    // tiles per XCD: consecutive tiles share a row panel in one L2
@@ -113,8 +114,13 @@ __host__ __device__ inline size_t stripe_off(size_t n, size_t S, size_t s) {
     return S * n * s - S * S * (s * (s - 1) / 2);
 }
 
+// Tiles of the stripes before stripe s: q row tiles per stripe, nbc column
+// tiles in stripe 0, qc fewer in each later stripe (S / tile width).
+__host__ __device__ inline long long tiles_before(long long s, long long q, long long qc, long long nbc) {
+    return q * (s * nbc - qc * (s * (s - 1) / 2));
+}
 __host__ __device__ inline long long tiles_before(long long s, long long q, long long nbc) {
-    return q * (s * nbc - q * (s * (s - 1) / 2));
+    return tiles_before(s, q, q, nbc);
 }
 
 // 32 bits of one row -> 16 int8 operand bytes of lane half g (0 or 1):
@@ -190,20 +196,23 @@ __device__ __forceinline__ Acc mfma_fp4(v8i a, v8i b, Acc c) {
 constexpr int KG = KC / 4;   // 4-word groups per slab
 constexpr int NST_DEFAULT = 3;  // LDS ring stages (slabs in flight); the autotuner also tries 2 and 4
 
-// ring[stage][side A/B][group][row] of 4 consecutive words: one
-// wavefront-wide direct-to-LDS load fills 64 consecutive rows of a group
-// (1 KiB, contiguous), and a wavefront's 16-byte reads of 32 consecutive
-// rows are contiguous, hence conflict-free.
-template <int BM, int NST>
-using Ring = uint4[NST][2][KG][BM];
+// ring.a[stage][group][row] (BM tile rows) and ring.b[stage][group][row] (BN
+// tile columns) of 4 consecutive words: one wavefront-wide direct-to-LDS load
+// fills 64 consecutive rows of a group (1 KiB, contiguous), and a wavefront's
+// 16-byte reads of 32 consecutive rows are contiguous, hence conflict-free.
+template <int BM, int BN, int NST>
+struct Ring {
+    uint4 a[NST][KG][BM];
+    uint4 b[NST][KG][BN];
+};
 
-// One BM x BM tile (I, J) of stripe s of job a, WM x WN per wavefront.
-template <int BM, int WM, int WN, bool FP4, int NST>
-__device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J, Ring<BM, NST>& ring) {
-    constexpr int WAVES_N = BM / WN;
+// One BM x BN tile (I, J) of stripe s of job a, WM x WN per wavefront.
+template <int BM, int BN, int WM, int WN, bool FP4, int NST>
+__device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J, Ring<BM, BN, NST>& ring) {
+    constexpr int WAVES_N = BN / WN;
     constexpr int NT = (BM / WM) * WAVES_N * 64;
     constexpr int TM = WM / 32, TN = WN / 32;
-    constexpr int WLOADS = 2 * KG * (BM / 64);  // wavefront-wide 16-byte loads per slab
+    constexpr int WLOADS = KG * ((BM + BN) / 64);  // wavefront-wide 16-byte loads per slab
     constexpr int LPW = WLOADS / (NT / 64);     // of which each wavefront issues
     static_assert(WLOADS % (NT / 64) == 0, "slab loads must divide evenly");
 
@@ -212,12 +221,12 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     const int rows = min(a.S, a.n - sr0), w = a.rect ? a.nB : a.n - sr0;
     uint16_t* __restrict__ out = a.rect ? a.out + (size_t)(sr0 - a.r0) * a.nB
                                         : a.out + stripe_off((size_t)(a.n - a.r0), (size_t)a.S, (size_t)s);
-    const int rt0 = I * BM, ct0 = J * BM;  // tile origin, stripe-relative
+    const int rt0 = I * BM, ct0 = J * BN;  // tile origin, stripe-relative
 
-    if (!a.rect && J < I) {  // entirely below the diagonal: every cell has j < i
-        const int rend = min(BM, rows - rt0), cend = min(BM, w - ct0);
-        for (int e = tid; e < rend * BM; e += NT) {
-            const int r = e / BM, c = e % BM;
+    if (!a.rect && ct0 + BN <= rt0) {  // entirely below the diagonal: every cell has j < i
+        const int rend = min(BM, rows - rt0), cend = min(BN, w - ct0);
+        for (int e = tid; e < rend * BN; e += NT) {
+            const int r = e / BN, c = e % BN;
             if (c < cend) out[(size_t)(rt0 + r) * w + ct0 + c] = 0;
         }
         return;
@@ -227,7 +236,15 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     const int wm0 = (wave / WAVES_N) * WM, wn0 = (wave % WAVES_N) * WN;
     const int l32 = lane & 31, g = lane >> 5;
 
-    const int nc1 = a.kw1 / KC, nc = nc1 + a.kw2 / KC;
+    // int8: the O slabs (kw1 / KC), then the P slabs.  fp4: one K sequence of
+    // 128-bit planes, the npO planes of O that can hold a set bit followed by
+    // the npP such planes of P, KG planes per slab (so no slab is padding
+    // between the two); a plane past the end reads some P plane and is not
+    // multiplied.
+    const int nc1 = a.kw1 / KC;
+    const int npO = ((a.kv1 ? a.kv1 : a.kw1) + 3) / 4, npP = ((a.kv2 ? a.kv2 : a.kw2) + 3) / 4;
+    const int npl = npO + npP;
+    const int nc = FP4 ? (npl + KG - 1) / KG : nc1 + a.kw2 / KC;
     const int growA = sr0 + rt0, growB = a.rect ? ct0 : sr0 + ct0;
 
     // Slab c (KC words of every A and B row of the tile) -> ring stage st,
@@ -236,26 +253,34 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     // n read row n-1: their products only reach output cells that are never
     // stored.
     auto issue_slab = [&](int c, int st) {
-        const uint32_t *baseA, *baseB;
-        int k0;
-        if (c < nc1) { baseA = a.O; baseB = a.OB; k0 = c * KC; }
-        else { baseA = a.P; baseB = a.PB; k0 = (c - nc1) * KC; }
 #pragma unroll
         for (int u = 0; u < LPW; u++) {
             const int v = wave * LPW + u;
-            const int side = v / (KG * (BM / 64));
-            const int rem = v % (KG * (BM / 64));
-            const int grp = rem / (BM / 64), rb = rem % (BM / 64);
+            const int side = v >= KG * (BM / 64);
+            const int rem = side ? v - KG * (BM / 64) : v;
+            const int nrb = side ? BN / 64 : BM / 64;
+            const int grp = rem / nrb, rb = rem % nrb;
+            bool isO;
+            int plane;
+            if (FP4) {
+                const int p = c * KG + grp;
+                isO = p < npO;
+                plane = isO ? p : min(p - npO, npP - 1);
+            } else {
+                isO = c < nc1;
+                plane = (isO ? c : c - nc1) * KG + grp;
+            }
+            const uint32_t* base = side ? (isO ? a.OB : a.PB) : (isO ? a.O : a.P);
             const int grow = side ? min(growB + rb * 64 + lane, a.nB - 1) : min(growA + rb * 64 + lane, a.n - 1);
-            const uint32_t* src =
-                (side ? baseB : baseA) + ((size_t)(k0 / 4 + grp) * (size_t)(side ? a.ldB : a.ldA) + grow) * 4;
+            const uint32_t* src = base + ((size_t)plane * (size_t)(side ? a.ldB : a.ldA) + grow) * 4;
             // LDS base of this wavefront's 64 rows goes to M0.  Written as an
             // instruction rather than the builtin so that the compiler does
             // not wait for every LDS-DMA transfer before each LDS read (it
             // cannot tell the ring stages apart); the waits are explicit
             // below.  The load has no register result.
             const uint32_t lds = __builtin_amdgcn_readfirstlane(
-                (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)&ring[st][side][grp][rb * 64]);
+                (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)(side ? &ring.b[st][grp][rb * 64]
+                                                                                     : &ring.a[st][grp][rb * 64]));
             asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(src), "s"(lds) : "memory", "m0");
         }
     };
@@ -281,40 +306,54 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
 
     // int8: acc accumulates O O^T - P P^T (O slabs with B = +O, P slabs with
     // B = -P, the short ones), negated in the epilogue.  fp4: acc holds
-    // O O^T, is negated once between the phases and then adds P P^T.
+    // O O^T over the O planes, is negated once at the first P plane and then
+    // adds P P^T.
     // Words of group gi+1 are read from LDS while group gi is multiplied.
     // fp4: each lane half g reads words 2g, 2g+1 of a 4-word group (one
     // 8-byte read), one word per 64-deep MFMA step.
-    auto compute_slab_fp4 = [&](int buf, auto neg) {
+    auto compute_slab_fp4 = [&](int buf, int c) {
         __builtin_amdgcn_iglp_opt(1);  // interleave the LDS reads and expansions with the MFMAs
         uint2 wa[TM], wb[TN];
 #pragma unroll
-        for (int i = 0; i < TM; i++) wa[i] = reinterpret_cast<const uint2*>(&ring[buf][0][0][wm0 + i * 32 + l32])[g];
+        for (int i = 0; i < TM; i++) wa[i] = reinterpret_cast<const uint2*>(&ring.a[buf][0][wm0 + i * 32 + l32])[g];
 #pragma unroll
-        for (int j = 0; j < TN; j++) wb[j] = reinterpret_cast<const uint2*>(&ring[buf][1][0][wn0 + j * 32 + l32])[g];
+        for (int j = 0; j < TN; j++) wb[j] = reinterpret_cast<const uint2*>(&ring.b[buf][0][wn0 + j * 32 + l32])[g];
 #pragma unroll
         for (int gi = 0; gi < KG; gi++) {
             uint2 na[TM], nb[TN];
             if (gi + 1 < KG) {
 #pragma unroll
                 for (int i = 0; i < TM; i++)
-                    na[i] = reinterpret_cast<const uint2*>(&ring[buf][0][gi + 1][wm0 + i * 32 + l32])[g];
+                    na[i] = reinterpret_cast<const uint2*>(&ring.a[buf][gi + 1][wm0 + i * 32 + l32])[g];
 #pragma unroll
                 for (int j = 0; j < TN; j++)
-                    nb[j] = reinterpret_cast<const uint2*>(&ring[buf][1][gi + 1][wn0 + j * 32 + l32])[g];
+                    nb[j] = reinterpret_cast<const uint2*>(&ring.b[buf][gi + 1][wn0 + j * 32 + l32])[g];
             }
-#pragma unroll
-            for (int kk = 0; kk < 2; kk++) {
-                v8i av[TM], bv[TN];
-#pragma unroll
-                for (int i = 0; i < TM; i++) av[i] = expand_fp4_a(kk ? wa[i].y : wa[i].x);
-#pragma unroll
-                for (int j = 0; j < TN; j++) bv[j] = expand_fp4_b(kk ? wb[j].y : wb[j].x);
+            // plane c * KG + gi (uniform): the first P plane negates acc,
+            // a plane past the end is skipped
+            const int p = c * KG + gi;
+            if (p == npO) {
 #pragma unroll
                 for (int i = 0; i < TM; i++)
 #pragma unroll
                     for (int j = 0; j < TN; j++)
-                        acc[i][j] = mfma_fp4(av[i], bv[j], acc[i][j]);
+#pragma unroll
+                        for (int e = 0; e < 16; e++) acc[i][j][e] = -acc[i][j][e];
+            }
+            if (p < npl) {
+#pragma unroll
+                for (int kk = 0; kk < 2; kk++) {
+                    v8i av[TM], bv[TN];
+#pragma unroll
+                    for (int i = 0; i < TM; i++) av[i] = expand_fp4_a(kk ? wa[i].y : wa[i].x);
+#pragma unroll
+                    for (int j = 0; j < TN; j++) bv[j] = expand_fp4_b(kk ? wb[j].y : wb[j].x);
+#pragma unroll
+                    for (int i = 0; i < TM; i++)
+#pragma unroll
+                        for (int j = 0; j < TN; j++)
+                            acc[i][j] = mfma_fp4(av[i], bv[j], acc[i][j]);
+                }
             }
             if (gi + 1 < KG) {
 #pragma unroll
@@ -327,17 +366,17 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     auto compute_slab_i8 = [&](int buf, auto neg) {
         uint4 wa[TM], wb[TN];
 #pragma unroll
-        for (int i = 0; i < TM; i++) wa[i] = ring[buf][0][0][wm0 + i * 32 + l32];
+        for (int i = 0; i < TM; i++) wa[i] = ring.a[buf][0][wm0 + i * 32 + l32];
 #pragma unroll
-        for (int j = 0; j < TN; j++) wb[j] = ring[buf][1][0][wn0 + j * 32 + l32];
+        for (int j = 0; j < TN; j++) wb[j] = ring.b[buf][0][wn0 + j * 32 + l32];
 #pragma unroll
         for (int gi = 0; gi < KG; gi++) {
             uint4 na[TM], nb[TN];
             if (gi + 1 < KG) {
 #pragma unroll
-                for (int i = 0; i < TM; i++) na[i] = ring[buf][0][gi + 1][wm0 + i * 32 + l32];
+                for (int i = 0; i < TM; i++) na[i] = ring.a[buf][gi + 1][wm0 + i * 32 + l32];
 #pragma unroll
-                for (int j = 0; j < TN; j++) nb[j] = ring[buf][1][gi + 1][wn0 + j * 32 + l32];
+                for (int j = 0; j < TN; j++) nb[j] = ring.b[buf][gi + 1][wn0 + j * 32 + l32];
             }
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) {
@@ -363,8 +402,8 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
             }
         }
     };
-    auto compute_slab = [&](int buf, auto neg) {
-        if constexpr (FP4) compute_slab_fp4(buf, neg);
+    auto compute_slab = [&](int buf, int c, auto neg) {
+        if constexpr (FP4) compute_slab_fp4(buf, c);
         else compute_slab_i8(buf, neg);
     };
 
@@ -377,19 +416,13 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
         // workgroup barrier (no memory fence: the DMA waits are explicit)
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" : : : "memory");
         issue_slab(min(c + NST - 1, nc - 1), (c + NST - 1) % NST);
-        compute_slab(c % NST, neg);
+        compute_slab(c % NST, c, neg);
     };
-    int c = 0;
-    for (; c < nc1; c++) step(c, std::false_type{});
     if constexpr (FP4) {
-#pragma unroll
-        for (int i = 0; i < TM; i++)
-#pragma unroll
-            for (int j = 0; j < TN; j++)
-#pragma unroll
-                for (int e = 0; e < 16; e++) acc[i][j][e] = -acc[i][j][e];
-        for (; c < nc; c++) step(c, std::false_type{});
+        for (int c = 0; c < nc; c++) step(c, std::false_type{});
     } else {
+        int c = 0;
+        for (; c < nc1; c++) step(c, std::false_type{});
         for (; c < nc; c++) step(c, std::true_type{});
     }
     // the trailing reloads must land before the workgroup's LDS is released
@@ -422,7 +455,7 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     // triangle, every column right of every row) need no per-cell test: one
     // convert and one store per cell, addressed as a wavefront-uniform base
     // plus a 32-bit lane offset; the row offsets are wavefront-uniform too.
-    const bool interior = rt0 + BM <= rows && ct0 + BM <= w && (a.rect || ct0 >= rt0 + BM);
+    const bool interior = rt0 + BM <= rows && ct0 + BN <= w && (a.rect || ct0 >= rt0 + BM);
     if (interior && a.cap <= 0) {
         char* __restrict__ wb0 = reinterpret_cast<char*>(out + (size_t)(rt0 + wm0) * w + (ct0 + wn0));
         const uint32_t lo = 2u * ((uint32_t)l32 + (uint32_t)(g * 4) * (uint32_t)w);  // byte offsets below 2^32
@@ -443,17 +476,20 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     }
 }
 
-// Tile shapes used: 256 x 256 by 8 wavefronts of 128 x 64 (two per SIMD, so
-// one wavefront's LDS and barrier waits are covered by the other's MFMAs) and
-// 128 x 128 by 4 wavefronts of 64 x 64.
+// Tile shapes of the closed-form launch: 256 x 128 by 4 wavefronts of
+// 128 x 64 with a 2-slab ring (48 KiB of LDS, at most 256 VGPRs), so that
+// two workgroups share a CU and one's tile start, barriers and epilogue are
+// covered by the other's MFMAs; and 128 x 128 by 4 wavefronts of 64 x 64.
+// The batch queue keeps 256 x 256 by 8 wavefronts of 128 x 64 (two per
+// SIMD).
 //
 // One launch over the closed-form tile numbering of TileArgs.  Workgroups
 // are dealt round-robin to the 8 XCDs; each XCD gets a contiguous run of
 // tiles so concurrently resident tiles share panels in that XCD's L2.
-template <int BM, int WM, int WN, bool FP4, int NST>
-__global__ void __launch_bounds__((BM / WM) * (BM / WN) * 64)
+template <int BM, int BN, int WM, int WN, bool FP4, int NST>
+__global__ void __launch_bounds__((BM / WM) * (BN / WN) * 64, BN < BM ? 2 : 1)
 mfma_tile(TileArgs a) {
-    __shared__ Ring<BM, NST> ring;
+    __shared__ Ring<BM, BN, NST> ring;
     const long long t = (long long)(blockIdx.x & 7) * a.chunk + (blockIdx.x >> 3);
     if (t >= a.T) return;
     int s = 0;
@@ -461,13 +497,13 @@ mfma_tile(TileArgs a) {
         int lo = 0, hi = a.ns - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (tiles_before(mid, a.q, a.nbc) <= t) lo = mid; else hi = mid - 1;
+            if (tiles_before(mid, a.q, a.qc, a.nbc) <= t) lo = mid; else hi = mid - 1;
         }
         s = lo;
     }
-    const long long local = t - (s ? tiles_before(s, a.q, a.nbc) : 0);
-    const int tn = a.nbc - s * a.q;
-    tile_body<BM, WM, WN, FP4, NST>(a.job, s, (int)(local / tn), (int)(local % tn), ring);
+    const long long local = t - (s ? tiles_before(s, a.q, a.qc, a.nbc) : 0);
+    const int tn = a.nbc - s * a.qc;
+    tile_body<BM, BN, WM, WN, FP4, NST>(a.job, s, (int)(local / tn), (int)(local % tn), ring);
 }
 
 // A job read from memory by a uniform index is uniform: say so, so that its
@@ -506,7 +542,7 @@ template <int BM, int WM, int WN, bool FP4>
 __global__ void __launch_bounds__((BM / WM) * (BM / WN) * 64)
 mfma_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* __restrict__ outoff,
                 const int4* __restrict__ tiles, int T, int* __restrict__ next) {
-    __shared__ Ring<BM, NST_DEFAULT> ring;
+    __shared__ Ring<BM, BM, NST_DEFAULT> ring;
     __shared__ int slot;
     for (;;) {
         if (threadIdx.x == 0) slot = atomicAdd(next, 1);
@@ -518,7 +554,7 @@ mfma_tile_queue(const TileJob* __restrict__ jobs, uint16_t* base, const size_t* 
         TileJob jb = jobs[tl.x];
         jb.out = base + outoff[tl.x];
         scalarize(jb);
-        tile_body<BM, WM, WN, FP4, NST_DEFAULT>(jb, tl.y, tl.z, tl.w, ring);
+        tile_body<BM, BM, WM, WN, FP4, NST_DEFAULT>(jb, tl.y, tl.z, tl.w, ring);
     }
 }
 
@@ -1220,8 +1256,14 @@ hipError_t upload_members(const uint32_t* mem, size_t count, bool m16, void** ou
     return e;
 }
 
-// One product launch over the tiles of `a` with the kernel form, tile edge and
-// LDS ring depth (2, 3 or 4 slabs) given; grid = a.chunk * 8 workgroups.
+// Column width of a product tile: the MFMA forms' 256-row tile is 256 x 128
+// (two workgroups per CU), every other tile is square.
+int tile_width(int kernel, int tile) { return tile == 256 && kernel != CGH_KERNEL_VALU_POPC ? 128 : tile; }
+
+// One product launch over the tiles of `a` with the kernel form, tile edge
+// (rows) and LDS ring depth given (2, 3 or 4 slabs for the 128-tile; the
+// 256 x 128 tile always runs a 2-slab ring, which keeps two workgroups per
+// CU); grid = a.chunk * 8 workgroups.
 hipError_t launch_tile_args(const TileArgs& a, int kernel, int tile, int nst, hipStream_t st) {
     const dim3 grid((unsigned)(a.chunk * 8));
     const bool fp4 = kernel == CGH_KERNEL_MFMA_FP4;
@@ -1230,18 +1272,17 @@ hipError_t launch_tile_args(const TileArgs& a, int kernel, int tile, int nst, hi
         else hipLaunchKernelGGL((valu_tile<128>), grid, dim3(256), 0, st, a);
         return hipGetLastError();
     }
-#define CGH_MFMA_LAUNCH(BM, WM, NT, F, K)                                                            \
-    hipLaunchKernelGGL((mfma_tile<BM, WM, 64, F, K>), grid, dim3(NT), 0, st, a)
-#define CGH_MFMA_STAGES(BM, WM, NT, F)           \
-    switch (nst) {                               \
-        case 2: CGH_MFMA_LAUNCH(BM, WM, NT, F, 2); break; \
-        case 4: CGH_MFMA_LAUNCH(BM, WM, NT, F, 4); break; \
-        default: CGH_MFMA_LAUNCH(BM, WM, NT, F, 3); break; \
+#define CGH_MFMA_LAUNCH(F, K) hipLaunchKernelGGL((mfma_tile<128, 128, 64, 64, F, K>), grid, dim3(256), 0, st, a)
+#define CGH_MFMA_STAGES(F)                         \
+    switch (nst) {                                 \
+        case 2: CGH_MFMA_LAUNCH(F, 2); break;      \
+        case 4: CGH_MFMA_LAUNCH(F, 4); break;      \
+        default: CGH_MFMA_LAUNCH(F, 3); break;     \
     }
-    if (tile == 256 && fp4) { CGH_MFMA_STAGES(256, 128, 512, true) }
-    else if (tile == 256) { CGH_MFMA_STAGES(256, 128, 512, false) }
-    else if (fp4) { CGH_MFMA_STAGES(128, 64, 256, true) }
-    else { CGH_MFMA_STAGES(128, 64, 256, false) }
+    if (tile == 256 && fp4) hipLaunchKernelGGL((mfma_tile<256, 128, 128, 64, true, 2>), grid, dim3(256), 0, st, a);
+    else if (tile == 256) hipLaunchKernelGGL((mfma_tile<256, 128, 128, 64, false, 2>), grid, dim3(256), 0, st, a);
+    else if (fp4) { CGH_MFMA_STAGES(true) }
+    else { CGH_MFMA_STAGES(false) }
 #undef CGH_MFMA_STAGES
 #undef CGH_MFMA_LAUNCH
     return hipGetLastError();
@@ -1250,6 +1291,7 @@ hipError_t launch_tile_args(const TileArgs& a, int kernel, int tile, int nst, hi
 hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int ld, int n,
                         int r0, int S, int ns, uint16_t* out, hipStream_t st, const uint32_t* rstart = NULL,
                         int cap = 0, int kv1 = 0, int kv2 = 0, int nst = NST_DEFAULT) {
+    const int tw = tile_width(kernel, tile);
     if (ns > 1 && S % tile) {
         for (int s = 0; s < ns; s++) {
             const hipError_t e = launch_tiles(kernel, tile, O, kw1, P, kw2, ld, n, r0 + s * S, S, 1,
@@ -1267,11 +1309,12 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
     a.job.rstart = rstart; a.job.cap = cap;
     a.job.kv1 = kv1; a.job.kv2 = kv2;
     a.ns = ns;
-    a.nbc = (n - r0 + tile - 1) / tile;
+    a.nbc = (n - r0 + tw - 1) / tw;
     a.q = S / tile;
+    a.qc = S / tw;
     const int last = r0 + (ns - 1) * S;
     const long long tm_last = (std::min(S, n - last) + tile - 1) / tile;
-    a.T = (ns > 1 ? tiles_before(ns - 1, a.q, a.nbc) : 0) + tm_last * (a.nbc - (long long)(ns - 1) * a.q);
+    a.T = (ns > 1 ? tiles_before(ns - 1, a.q, a.qc, a.nbc) : 0) + tm_last * (a.nbc - (long long)(ns - 1) * a.qc);
     a.chunk = (a.T + 7) / 8;
     return launch_tile_args(a, kernel, tile, nst, st);
 }
@@ -1546,10 +1589,10 @@ extern "C" void cgh_dataset_tuning(const cgh_dataset* ds, int* kernel, int* tile
 
 extern "C" int cgh_dataset_autotune(cgh_dataset* ds, int S, int* kernel_out, int* tile_out, double* ms_out,
                                     char* err, int errlen) {
-    // Candidates: fp4 and int8 MFMA x 128- and 256-tiles x LDS rings of 2,
-    // 3 and 4 slabs (the VALU form is never faster), each timed on the first
-    // stripe (S rows, the widest) after one warm-up launch; the fastest is
-    // kept.  Every candidate's stripe is compared with the first one's: a
+    // Candidates: fp4 and int8 MFMA x the 256 x 128 tile (its 2-slab ring)
+    // and the 128-tile with LDS rings of 2, 3 and 4 slabs (the VALU form is
+    // never faster), each timed on the first stripe (S rows, the widest)
+    // after one warm-up launch; the fastest is kept.  Every candidate's stripe is compared with the first one's: a
     // candidate that disagrees is an error, not a choice.
     const int n = ds->n;
     const int ns = 1;
@@ -1576,6 +1619,7 @@ extern "C" int cgh_dataset_autotune(cgh_dataset* ds, int S, int* kernel_out, int
     for (int k : kernels)
         for (int t : tiles)
             for (int g : stages) {
+                if (t == 256 && g != 2) continue;  // one ring depth for the 256 x 128 tile
                 float ms = 0.f;
                 int diff = 0;
                 HIP_TRY(launch_tiles(k, t, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0, S, ns, buf,
@@ -2490,12 +2534,14 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     TileArgs a;
     a.job = {sl.dO, sl.dP, (int)db->ds->kw1, (int)db->ds->kw2, q, 0, q, sl.ddist, db->ds->dO, db->ds->dP, db->n, 1,
              q_pad, (int)db->ds->n_pad, NULL, 0, ds_kv1(db->ds), ds_kv2(db->ds)};
+    const int kernel = resolve_kernel(sv->kernel), tw = tile_width(kernel, tile);
     a.ns = 1;
     a.q = 0;
-    a.nbc = (db->n + tile - 1) / tile;
+    a.qc = 0;
+    a.nbc = (db->n + tw - 1) / tw;
     a.T = (long long)((q + tile - 1) / tile) * a.nbc;
     a.chunk = (a.T + 7) / 8;
-    if ((e = launch_tile_args(a, resolve_kernel(sv->kernel), tile, NST_DEFAULT, sl.st))) return e;
+    if ((e = launch_tile_args(a, kernel, tile, NST_DEFAULT, sl.st))) return e;
     if (ne) {
         RareJob jb = rare_job(sl.ddist, q, 0, q, sl.drs, sl.dre, db->dqmem, db->m16, 0, 32);
         jb.rect = 1;

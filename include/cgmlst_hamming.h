@@ -171,8 +171,9 @@ int cgh_dataset_encode(int device, const uint64_t* calls, int n, int L, int tau,
  * cap < 0. */
 int cgh_dataset_set_cap(cgh_dataset* ds, int cap);
 
-/* Autotuner: time the product kernel candidates (fp4 and int8 MFMA, 128- and
- * 256-tiles, LDS rings of 2, 3 and 4 slabs: 12 configurations) on the
+/* Autotuner: time the product kernel candidates (fp4 and int8 MFMA x the
+ * 256-row tile, 256 x 128 with a 2-slab LDS ring, and the 128 x 128 tile with
+ * rings of 2, 3 and 4 slabs: 8 configurations) on the
  * dataset's first stripe of S rows and keep the fastest for CGH_KERNEL_AUTO
  * in every later call on this dataset (upper, phase, stream; a batch is
  * tuned as a whole by cgh_batch_autotune).  The stripe height is not a
