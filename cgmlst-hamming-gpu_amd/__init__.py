@@ -28,7 +28,7 @@ EXPORTS = (
     "cgh_dense_ranks", "cgh_encode", "cgh_encode_meta", "cgh_column_table", "cgh_rare_csr", "cgh_encoding_free",
     "cgh_allele_counts", "cgh_choose_tau",
     "cgh_dataset_upload", "cgh_dataset_build", "cgh_dataset_encode", "cgh_dataset_set_cap", "cgh_dataset_autotune", "cgh_dataset_info", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
-    "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream",
+    "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream", "cgh_dataset_stream_graph",
     "cgh_batch_create", "cgh_batch_out_elems", "cgh_batch_out_offset", "cgh_batch_tile", "cgh_batch_run",
     "cgh_batch_free",
     "cgh_codec_create", "cgh_codec_info", "cgh_codec_members", "cgh_codec_encode", "cgh_codec_free",
@@ -97,6 +97,8 @@ def load():
         "cgh_dataset_upper": (I, [P, I, P, P, I, C.c_char_p, I]),
         "cgh_dataset_phase": (I, [P, I, P, P, I, I, C.c_char_p, I]),
         "cgh_dataset_stream": (I, [P, I, TCH_CB, P, C.POINTER(C.c_double), C.c_char_p, I]),
+        "cgh_dataset_stream_graph": (I, [P, I, TCH_CB, P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_char_p,
+                                         I]),
         "cgh_measure_peaks": (I, [I] + [C.POINTER(C.c_double)] * 4 + [C.c_char_p, I]),
         "cgh_batch_create": (I, [I, C.POINTER(P), I, I, I, C.POINTER(P), C.c_char_p, I]),
         "cgh_batch_out_elems": (S, [P]),
@@ -340,8 +342,9 @@ class Dataset:
 
     __del__ = close
 
-    def stream(self, S: int):
-        """cgh_dataset_stream into a square symmetric H (host)."""
+    def stream(self, S: int, graph: bool = False):
+        """cgh_dataset_stream (graph: cgh_dataset_stream_graph) into a square
+        symmetric H (host)."""
         n = self.n
         H = np.zeros((n, n), dtype=np.uint16)
         seen = []
@@ -353,7 +356,12 @@ class Dataset:
         ccb = TCH_CB(cb)
         secs = C.c_double()
         err = _err()
-        _check(load().cgh_dataset_stream(self.handle, S, ccb, None, C.byref(secs), err, 512), err)
+        if graph:
+            build = C.c_double()
+            _check(load().cgh_dataset_stream_graph(self.handle, S, ccb, None, C.byref(secs), C.byref(build), err, 512),
+                   err)
+        else:
+            _check(load().cgh_dataset_stream(self.handle, S, ccb, None, C.byref(secs), err, 512), err)
         iu = np.triu_indices(n, 1)
         H.T[iu] = H[iu]
         np.fill_diagonal(H, 0)

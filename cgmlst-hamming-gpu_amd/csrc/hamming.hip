@@ -2822,14 +2822,17 @@ public:
         // in-flight copies and kernels still reference the buffers
         if (compute_) (void)hipStreamSynchronize(compute_);
         if (copy_) (void)hipStreamSynchronize(copy_);
+        if (host_stream_) (void)hipStreamSynchronize(host_stream_);
         for (int k = 0; k < SLOTS; k++) {
             if (dev_[k]) (void)hipFree(dev_[k]);
             if (host_[k]) (void)hipHostFree(host_[k]);
             if (computed_[k]) (void)hipEventDestroy(computed_[k]);
             if (landed_[k]) (void)hipEventDestroy(landed_[k]);
+            if (consumed_[k]) (void)hipEventDestroy(consumed_[k]);
         }
         if (compute_) (void)hipStreamDestroy(compute_);
         if (copy_) (void)hipStreamDestroy(copy_);
+        if (host_stream_) (void)hipStreamDestroy(host_stream_);
     }
 
     // Buffers of `cells` u16 per slot (a multiple of 8: the rare pass stages
@@ -2837,11 +2840,13 @@ public:
     hipError_t open(size_t cells) {
         hipError_t e = hipStreamCreateWithFlags(&compute_, hipStreamNonBlocking);
         if (!e) e = hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking);
+        if (!e) e = hipStreamCreateWithFlags(&host_stream_, hipStreamNonBlocking);
         for (int k = 0; k < SLOTS && !e; k++) {
             e = hipMalloc(&dev_[k], cells * 2);
             if (!e) e = hipHostMalloc(&host_[k], cells * 2, hipHostMallocDefault);
             if (!e) e = hipEventCreateWithFlags(&computed_[k], hipEventDisableTiming);
             if (!e) e = hipEventCreateWithFlags(&landed_[k], hipEventDisableTiming);
+            if (!e) e = hipEventCreateWithFlags(&consumed_[k], hipEventDisableTiming);
         }
         return e;
     }
@@ -2874,13 +2879,89 @@ public:
         return e;
     }
 
+    // The same pipeline captured as one HIP graph and launched once: per
+    // stripe a compute node sequence into device slot s % SLOTS, a copy node
+    // to pinned host slot s % SLOTS, and a host node that calls
+    // cb(ctx, r0, rows, w, cells); the host nodes run in stripe order, and
+    // the compute of stripe s + SLOTS depends on the host node of stripe s,
+    // so a slot is refilled only after its consumer has returned.  The
+    // callback runs on a runtime thread while the caller waits, one stripe
+    // at a time; it must not call HIP.  *build_ms gets the capture and
+    // instantiation time.
+    template <class Produce>
+    hipError_t run_graph(int n, int S, Produce&& produce, tch_cb cb, void* ctx, double* build_ms) {
+        struct HostCall {
+            tch_cb cb;
+            void* ctx;
+            int r0, rows;
+            size_t w;
+            const uint16_t* cells;
+        };
+        const int stripes = (n + S - 1) / S;
+        std::vector<HostCall> calls((size_t)stripes);
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t fork = nullptr, copied = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
+        hipError_t e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+        if (!e) e = hipEventCreateWithFlags(&copied, hipEventDisableTiming);
+        if (!e) e = hipStreamBeginCapture(compute_, hipStreamCaptureModeThreadLocal);
+        if (!e) {
+            // the copy and host streams join the capture through events
+            hipError_t c = hipEventRecord(fork, compute_);
+            if (!c) c = hipStreamWaitEvent(copy_, fork, 0);
+            if (!c) c = hipStreamWaitEvent(host_stream_, fork, 0);
+            for (int s = 0; s < stripes && !c; s++) {
+                const int k = s % SLOTS, r0 = s * S, rows = std::min(S, n - r0);
+                const size_t w = (size_t)(n - r0), cells = (size_t)rows * w;
+                calls[(size_t)s] = HostCall{cb, ctx, r0, rows, w, host_[k]};
+                if (s >= SLOTS) c = hipStreamWaitEvent(compute_, consumed_[k], 0);
+                if (!c) c = produce(r0, rows, dev_[k], compute_);
+                if (!c) c = hipEventRecord(computed_[k], compute_);
+                if (!c) c = hipStreamWaitEvent(copy_, computed_[k], 0);
+                if (!c) c = hipMemcpyAsync(host_[k], dev_[k], cells * 2, hipMemcpyDeviceToHost, copy_);
+                if (!c) c = hipEventRecord(landed_[k], copy_);
+                if (!c) c = hipStreamWaitEvent(host_stream_, landed_[k], 0);
+                if (!c)
+                    c = hipLaunchHostFunc(
+                        host_stream_,
+                        [](void* p) {
+                            const HostCall* h = static_cast<const HostCall*>(p);
+                            h->cb(h->ctx, h->r0, h->rows, h->w, h->cells);
+                        },
+                        &calls[(size_t)s]);
+                if (!c) c = hipEventRecord(consumed_[k], host_stream_);
+            }
+            // join both side streams back before the capture ends
+            if (!c) c = hipEventRecord(copied, copy_);
+            if (!c) c = hipStreamWaitEvent(compute_, copied, 0);
+            if (!c) c = hipStreamWaitEvent(compute_, consumed_[(stripes - 1) % SLOTS], 0);
+            e = hipStreamEndCapture(compute_, &graph);
+            if (c) e = c;
+        }
+        if (!e) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (build_ms) *build_ms = seconds(t0) * 1e3;
+        if (!e) e = hipGraphLaunch(exec, compute_);
+        if (!e) e = hipStreamSynchronize(compute_);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (fork) (void)hipEventDestroy(fork);
+        if (copied) (void)hipEventDestroy(copied);
+        return e;
+    }
+
 private:
+    static double seconds(std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    }
     uint16_t* dev_[SLOTS] = {};
     uint16_t* host_[SLOTS] = {};
     hipEvent_t computed_[SLOTS] = {};
     hipEvent_t landed_[SLOTS] = {};
+    hipEvent_t consumed_[SLOTS] = {};
     hipStream_t compute_ = nullptr;
     hipStream_t copy_ = nullptr;
+    hipStream_t host_stream_ = nullptr;
 };
 
 double seconds_since(std::chrono::steady_clock::time_point t) {
@@ -3023,31 +3104,62 @@ fail:
     return 1;
 }
 
+namespace {
+// The device work of one stripe of a dataset: one product launch (with the
+// cap of rows that have no rare partners) and one row-local rare launch over
+// the dataset's groups, into a rows x (n - r0) buffer.
+struct DatasetStripes {
+    cgh_dataset* ds;
+    int S, kernel, tile, stages, rsg;
+    explicit DatasetStripes(cgh_dataset* d, int s)
+        : ds(d), S(s), kernel(ds_kernel(d, CGH_KERNEL_AUTO)), tile(ds_tile(d, CGH_KERNEL_AUTO)),
+          stages(ds_stages(d, CGH_KERNEL_AUTO)), rsg(lanes_per_range(d->n_rare, d->n_rentries)) {}
+    hipError_t operator()(int r0, int rows, uint16_t* dst, hipStream_t st) const {
+        const int n = ds->n;
+        hipError_t e = launch_tiles(kernel, tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, r0, S,
+                                    1, dst, st, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds), stages);
+        if (!e && ds->n_rare && n - r0 > 1) {
+            RareJob jb = rare_job(dst, n, r0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap, rsg);
+            e = launch_rare(jb, rare_layout(jb, rows), st);
+        }
+        return e;
+    }
+};
+}  // namespace
+
 // The CLI's path: stripes of a device-resident dataset through the
-// StripeRing, each stripe one product launch (with the cap of rows that have
-// no rare partners) and one row-local rare launch over the dataset's groups.
+// StripeRing, each stripe's device work enqueued by the calling thread and
+// its callback run there.
 extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, char* err,
                                   int errlen) {
     if (!stripe_height_ok(S, err, errlen)) return 1;
     const auto t_start = std::chrono::steady_clock::now();
-    const int n = ds->n, kernel = ds_kernel(ds, CGH_KERNEL_AUTO), tile = ds_tile(ds, CGH_KERNEL_AUTO);
-    const int stages = ds_stages(ds, CGH_KERNEL_AUTO);
-    const int rsg = lanes_per_range(ds->n_rare, ds->n_rentries);
     StripeRing ring;
     HIP_TRY(hipSetDevice(ds->device));
-    HIP_TRY(ring.open(up((size_t)S * (size_t)n, 8)));
+    HIP_TRY(ring.open(up((size_t)S * (size_t)ds->n, 8)));
     {
-        auto produce = [&](int r0, int rows, uint16_t* dst, hipStream_t st) -> hipError_t {
-            hipError_t e = launch_tiles(kernel, tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n,
-                                        r0, S, 1, dst, st, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds), stages);
-            if (!e && ds->n_rare && n - r0 > 1) {
-                RareJob jb = rare_job(dst, n, r0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap, rsg);
-                e = launch_rare(jb, rare_layout(jb, rows), st);
-            }
-            return e;
-        };
+        const DatasetStripes produce(ds, S);
         auto consume = [&](int r0, int rows, size_t w, const uint16_t* cells) { cb(ctx, r0, rows, w, cells); };
-        HIP_TRY(ring.run(n, S, produce, consume));
+        HIP_TRY(ring.run(ds->n, S, produce, consume));
+    }
+    if (gpu_secs) *gpu_secs = seconds_since(t_start);
+    return 0;
+fail:
+    return 1;
+}
+
+// The same stripes as one captured HIP graph (StripeRing::run_graph).
+extern "C" int cgh_dataset_stream_graph(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs,
+                                        double* build_ms, char* err, int errlen) {
+    if (!stripe_height_ok(S, err, errlen)) return 1;
+    const auto t_start = std::chrono::steady_clock::now();
+    StripeRing ring;
+    HIP_TRY(hipSetDevice(ds->device));
+    HIP_TRY(ring.open(up((size_t)S * (size_t)ds->n, 8)));
+    (void)rare_grid(1);  // the CU count is looked up before the capture, not inside it
+    {
+        const DatasetStripes produce(ds, S);
+        HIP_TRY(ring.run_graph(ds->n, S, produce, cb, ctx, build_ms));
     }
     if (gpu_secs) *gpu_secs = seconds_since(t_start);
     return 0;

@@ -228,6 +228,17 @@ int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int
  * no counterpart in the reference beyond tch_run's shape. */
 int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, char* err, int errlen);
 
+/* The stripes of cgh_dataset_stream captured as one HIP graph and launched
+ * once: per stripe the product and rare launches, the copy to pinned host
+ * memory and a host node that calls cb, with the same three-buffer ring (the
+ * compute of stripe s + 3 depends on the host node of stripe s).  Same
+ * output and stripe order; the callback runs on a HIP runtime thread, one
+ * stripe at a time while the caller waits, and must not call HIP.
+ * *build_ms (may be NULL) gets the capture and instantiation time.  No
+ * counterpart in the reference (its stripe loop is eager, tch.cu:113-149). */
+int cgh_dataset_stream_graph(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, double* build_ms,
+                             char* err, int errlen);
+
 /* ------------------------------------------------------------------------ */
 /* Many independent datasets in one persistent launch (no counterpart in the */
 /* reference, which runs one dataset per process).                          */

@@ -198,15 +198,17 @@ def test_tch_run_rows_wider_than_one_chunk(cgh, gpu):
         assert np.array_equal(got[i], oracle.brute_rows(X, i, i + 1)[0, i + 1:]), i
 
 
+@pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("S,tau", [(64, 8), (520, 64), (4096, 1024), (8, 2)])
-def test_dataset_stream(cgh, gpu, workloads, S, tau):
-    """cgh_dataset_stream (the CLI's path): stripes in order, every cell."""
+def test_dataset_stream(cgh, gpu, workloads, S, tau, graph):
+    """cgh_dataset_stream (the CLI's path) and its HIP-graph form: stripes in
+    order, every cell."""
     calls = workloads.small_zipf(1500, 700, seed=9)
     X = cgh.dense_ranks(calls)
     enc = cgh.Encoded(X, tau, 256)
     ds = cgh.Dataset(enc)
-    print(f"dataset_stream S={S} tau={tau}", flush=True)
-    H, secs, seen = ds.stream(S)
+    print(f"dataset_stream S={S} tau={tau} graph={graph}", flush=True)
+    H, secs, seen = ds.stream(S, graph)
     ds.close()
     assert [s[0] for s in seen] == list(range(0, 1500, S))
     assert np.array_equal(H, oracle.brute_full(X))
