@@ -27,7 +27,7 @@ EXPORTS = (
     "tch_device_count", "tch_free_vram", "tch_run",
     "cgh_dense_ranks", "cgh_encode", "cgh_encode_meta", "cgh_column_table", "cgh_rare_csr", "cgh_encoding_free",
     "cgh_allele_counts", "cgh_choose_tau",
-    "cgh_dataset_upload", "cgh_dataset_build", "cgh_dataset_encode", "cgh_dataset_set_cap", "cgh_dataset_autotune", "cgh_dataset_info", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
+    "cgh_dataset_upload", "cgh_dataset_upload_split", "cgh_dataset_build", "cgh_dataset_encode", "cgh_dataset_set_cap", "cgh_dataset_autotune", "cgh_dataset_info", "cgh_dataset_free", "cgh_upper_elems", "cgh_stripe_offset",
     "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream", "cgh_dataset_stream_graph",
     "cgh_batch_create", "cgh_batch_out_elems", "cgh_batch_out_offset", "cgh_batch_tile", "cgh_batch_run",
     "cgh_batch_free",
@@ -91,6 +91,7 @@ def load():
         "cgh_allele_counts": (S, [P, I, I, P, S]),
         "cgh_choose_tau": (I, [P, S, I, I, S, S, LOG_CB, P]),
         "cgh_dataset_upload": (I, [I, C.POINTER(Encoding), C.POINTER(P), C.c_char_p, I]),
+        "cgh_dataset_upload_split": (I, [I, C.POINTER(Encoding), S, C.POINTER(P), C.c_char_p, I]),
         "cgh_dataset_free": (None, [P]),
         "cgh_upper_elems": (S, [I, I]),
         "cgh_stripe_offset": (S, [I, I, I]),
@@ -276,13 +277,19 @@ def stripe_offset(n: int, S: int, s: int) -> int:
 class Dataset:
     """Device-resident operands (cgh_dataset_upload)."""
 
-    def __init__(self, enc: Encoded = None, device: int = 0):
+    def __init__(self, enc: Encoded = None, device: int = 0, o_bytes: int = 0):
+        """o_bytes > 0: cgh_dataset_upload_split (the one-hot stays in host
+        memory and is streamed through o_bytes of device buffers)."""
         self.handle = C.c_void_p()
         if enc is None:
             return
         self.n = enc.enc.n
         err = _err()
-        _check(load().cgh_dataset_upload(device, C.byref(enc.enc), C.byref(self.handle), err, 512), err)
+        if o_bytes:
+            _check(load().cgh_dataset_upload_split(device, C.byref(enc.enc), o_bytes, C.byref(self.handle), err, 512),
+                   err)
+        else:
+            _check(load().cgh_dataset_upload(device, C.byref(enc.enc), C.byref(self.handle), err, 512), err)
 
     @classmethod
     def build(cls, X: np.ndarray, tau: int, device: int = 0):

@@ -90,6 +90,11 @@ struct TileJob {
     // the padding after them is zero, and the VALU form skips its planes.
     // 0 = unknown (every word).
     int kv1, kv2;
+    // MFMA forms, symmetric: add the tile's value to the stored cells j > i
+    // instead of storing it (a later pass of a one-hot split along K; cells
+    // below the diagonal are left alone).  With kw2 = 0 (no P planes) the
+    // value is -O O^T.
+    int accum;
 };
 
 // Cell value of row i (global) in the epilogue, capped unless the rare pass
@@ -224,6 +229,7 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     const int rt0 = I * BM, ct0 = J * BN;  // tile origin, stripe-relative
 
     if (!a.rect && ct0 + BN <= rt0) {  // entirely below the diagonal: every cell has j < i
+        if (a.accum) return;           // written as 0 by the first pass
         const int rend = min(BM, rows - rt0), cend = min(BN, w - ct0);
         for (int e = tid; e < rend * BN; e += NT) {
             const int r = e / BN, c = e % BN;
@@ -264,8 +270,8 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
             int plane;
             if (FP4) {
                 const int p = c * KG + grp;
-                isO = p < npO;
-                plane = isO ? p : min(p - npO, npP - 1);
+                isO = p < npO || npP == 0;  // with no P planes a plane past the end reads O plane 0
+                plane = p < npO ? p : npP ? min(p - npO, npP - 1) : 0;
             } else {
                 isO = c < nc1;
                 plane = (isO ? c : c - nc1) * KG + grp;
@@ -427,6 +433,14 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     }
     // the trailing reloads must land before the workgroup's LDS is released
     __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
+    if (FP4 && npP == 0) {  // O planes only: the value is -O O^T, as in the int8 form
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[i][j][e] = -acc[i][j][e];
+    }
 
     // Epilogue: lane holds column l32 and rows 8*(e/4) + 4*g + (e%4) of the
     // distance (fp4) or of its negation (int8).  The capped form is a
@@ -451,6 +465,32 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
                 }
             }
     };
+    // A later pass of a one-hot split along K: the value is added to the
+    // stored cell (it only ever lowers it towards the final distance); the
+    // cap goes with the last pass.
+    auto add_all = [&](auto capped) {
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                const int c = ct0 + wn0 + j * 32 + l32;
+                if (c >= w) continue;
+#pragma unroll
+                for (int e = 0; e < 16; e++) {
+                    const int r = rt0 + wm0 + i * 32 + (e >> 2) * 8 + g * 4 + (e & 3);
+                    if (r >= rows || c <= r) continue;
+                    const uint32_t v = FP4 ? (uint32_t)(int)acc[i][j][e] : (uint32_t)(-(int)acc[i][j][e]);
+                    uint16_t* cell = out + (size_t)r * w + c;
+                    const uint32_t nv = (uint32_t)*cell + v;
+                    *cell = decltype(capped)::value ? epilogue_cap(a, sr0 + r, nv) : (uint16_t)nv;
+                }
+            }
+    };
+    if (a.accum) {
+        if (a.cap > 0) add_all(std::true_type{});
+        else add_all(std::false_type{});
+        return;
+    }
     // Interior tiles (every row < rows, every column < w and, in the
     // triangle, every column right of every row) need no per-cell test: one
     // convert and one store per cell, addressed as a wavefront-uniform base
@@ -1290,13 +1330,13 @@ hipError_t launch_tile_args(const TileArgs& a, int kernel, int tile, int nst, hi
 
 hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const uint32_t* P, int kw2, int ld, int n,
                         int r0, int S, int ns, uint16_t* out, hipStream_t st, const uint32_t* rstart = NULL,
-                        int cap = 0, int kv1 = 0, int kv2 = 0, int nst = NST_DEFAULT) {
+                        int cap = 0, int kv1 = 0, int kv2 = 0, int nst = NST_DEFAULT, int accum = 0) {
     const int tw = tile_width(kernel, tile);
     if (ns > 1 && S % tile) {
         for (int s = 0; s < ns; s++) {
             const hipError_t e = launch_tiles(kernel, tile, O, kw1, P, kw2, ld, n, r0 + s * S, S, 1,
                                               out + stripe_off((size_t)(n - r0), (size_t)S, (size_t)s), st, rstart,
-                                              cap, kv1, kv2, nst);
+                                              cap, kv1, kv2, nst, accum);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -1308,6 +1348,7 @@ hipError_t launch_tiles(int kernel, int tile, const uint32_t* O, int kw1, const 
     a.job.ldA = ld; a.job.ldB = ld;
     a.job.rstart = rstart; a.job.cap = cap;
     a.job.kv1 = kv1; a.job.kv2 = kv2;
+    a.job.accum = accum;
     a.ns = ns;
     a.nbc = (n - r0 + tw - 1) / tw;
     a.q = S / tile;
@@ -1349,6 +1390,23 @@ extern "C" size_t tch_free_vram(int device) {
 // Device-resident dataset
 // ---------------------------------------------------------------------------
 
+namespace {
+// f(lo, hi) over [0, count) split into contiguous ranges on the host's
+// hardware threads.
+template <class F>
+void host_parallel(size_t count, F&& f) {
+    const size_t threads = std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), 64));
+    const size_t per = (count + threads - 1) / threads;
+    if (threads == 1 || count < 2 * threads) {
+        f((size_t)0, count);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t lo = 0; lo < count; lo += per) pool.emplace_back([&f, lo, per, count] { f(lo, std::min(count, lo + per)); });
+    for (auto& t : pool) t.join();
+}
+}  // namespace
+
 struct cgh_dataset {
     int device;
     int n, L, tau;
@@ -1364,11 +1422,28 @@ struct cgh_dataset {
     int m16;
     uint32_t* drstart;
     uint64_t* drentry;
+    // One-hot kept in host memory (cgh_dataset_upload_split): hO holds the
+    // kw1 / 4 O planes of n_pad rows in pinned memory, streamed through two
+    // device buffers of chunk_planes planes each (dO is NULL).
+    uint32_t* hO;
+    uint32_t* dOc[2];
+    size_t chunk_planes;
+    hipStream_t ostream;
+    hipEvent_t loaded[2], used[2];
 };
 
 extern "C" void cgh_dataset_free(cgh_dataset* ds) {
     if (!ds) return;
     (void)hipSetDevice(ds->device);
+    if (ds->ostream) (void)hipStreamSynchronize(ds->ostream);
+    for (int k = 0; k < 2; k++) {
+        if (ds->used[k]) (void)hipEventSynchronize(ds->used[k]);
+        if (ds->dOc[k]) (void)hipFree(ds->dOc[k]);
+        if (ds->loaded[k]) (void)hipEventDestroy(ds->loaded[k]);
+        if (ds->used[k]) (void)hipEventDestroy(ds->used[k]);
+    }
+    if (ds->ostream) (void)hipStreamDestroy(ds->ostream);
+    if (ds->hO) (void)hipHostFree(ds->hO);
     if (ds->dO) (void)hipFree(ds->dO);
     if (ds->dP) (void)hipFree(ds->dP);
     if (ds->dgmem) (void)hipFree(ds->dgmem);
@@ -1428,6 +1503,44 @@ extern "C" int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_datas
     *out = NULL;
     if (dataset_meta(device, enc, &ds, err, errlen)) return 1;
     HIP_TRY(upload_planes(enc->O, (int)enc->n_pad, (int)enc->kw1, (int)ds->kw1, &ds->dO));
+    HIP_TRY(upload_planes(enc->P, (int)enc->n_pad, (int)enc->kw2, (int)ds->kw2, &ds->dP));
+    *out = ds;
+    return 0;
+fail:
+    cgh_dataset_free(ds);
+    return 1;
+}
+
+extern "C" int cgh_dataset_upload_split(int device, const cgh_encoding* enc, size_t o_bytes, cgh_dataset** out,
+                                        char* err, int errlen) {
+    cgh_dataset* ds = NULL;
+    *out = NULL;
+    if (dataset_meta(device, enc, &ds, err, errlen)) return 1;
+    {
+        const size_t n_pad = ds->n_pad, planes = ds->kw1 / 4, plane_bytes = n_pad * 16;
+        // two chunk buffers, whole slabs (KG planes) each
+        ds->chunk_planes = std::min(o_bytes / 2 / plane_bytes / KG * KG, planes);
+        if (ds->chunk_planes < (size_t)KG) {
+            set_err(err, errlen, "cgh_dataset_upload_split: %zu bytes hold less than two slabs of %zu one-hot planes",
+                    o_bytes, planes);
+            cgh_dataset_free(ds);
+            return 1;
+        }
+        HIP_TRY(hipHostMalloc(&ds->hO, planes * plane_bytes, hipHostMallocDefault));
+        // row-major words (kw1 per row) -> planes of 4 words; planes past the
+        // encoding's words stay zero
+        memset(ds->hO, 0, planes * plane_bytes);
+        host_parallel((size_t)enc->n_pad, [&](size_t lo, size_t hi) {
+            for (size_t r = lo; r < hi; r++)
+                for (size_t w = 0; w < enc->kw1; w++) ds->hO[((w / 4) * n_pad + r) * 4 + w % 4] = enc->O[r * enc->kw1 + w];
+        });
+        for (int k = 0; k < 2; k++) {
+            HIP_TRY(hipMalloc(&ds->dOc[k], ds->chunk_planes * plane_bytes));
+            HIP_TRY(hipEventCreateWithFlags(&ds->loaded[k], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ds->used[k], hipEventDisableTiming));
+        }
+        HIP_TRY(hipStreamCreateWithFlags(&ds->ostream, hipStreamNonBlocking));
+    }
     HIP_TRY(upload_planes(enc->P, (int)enc->n_pad, (int)enc->kw2, (int)ds->kw2, &ds->dP));
     *out = ds;
     return 0;
@@ -1579,6 +1692,15 @@ int ds_stages(const cgh_dataset* ds, int kernel) {
 
 }  // namespace
 
+namespace {
+// The entry points that need O on the device refuse a split dataset.
+bool split_unsupported(const cgh_dataset* ds, const char* what, char* err, int errlen) {
+    if (!ds->hO) return false;
+    set_err(err, errlen, "%s: not available for a dataset whose one-hot stays in host memory", what);
+    return true;
+}
+}  // namespace
+
 extern "C" int cgh_dataset_tile(const cgh_dataset* ds, int kernel) { return ds_tile(ds, kernel); }
 
 extern "C" void cgh_dataset_tuning(const cgh_dataset* ds, int* kernel, int* tile, int* stages) {
@@ -1608,6 +1730,7 @@ extern "C" int cgh_dataset_autotune(cgh_dataset* ds, int S, int* kernel_out, int
         set_err(err, errlen, "stripe height must be a positive multiple of 8 (got %d)", S);
         return 1;
     }
+    if (split_unsupported(ds, "cgh_dataset_autotune", err, errlen)) return 1;
     HIP_TRY(hipSetDevice(ds->device));
     HIP_TRY(hipMalloc(&buf, bytes));
     HIP_TRY(hipMalloc(&first, bytes));
@@ -1672,6 +1795,46 @@ out:
     return cand == 0 ? 0 : 1;
 }
 
+namespace {
+// The product of a dataset whose one-hot stays in host memory: chunk c of
+// chunk_planes O planes goes up on the dataset's upload stream into buffer
+// c % 2 (after pass c - 2 has read that buffer), and pass c multiplies it on
+// `st`.  Pass 0 also takes the P planes and stores P P^T - O_0 O_0^T; each
+// later pass adds -O_c O_c^T to the stored cells, so every cell only falls
+// towards P P^T - O O^T (never below it: no u16 underflow); the cap goes
+// with the last pass.  The uploads of chunk c + 1 overlap pass c.
+int split_passes(cgh_dataset* ds, int S, uint16_t* d_out, hipStream_t st, int kernel, int tile, char* err,
+                 int errlen) {
+    if (kernel != CGH_KERNEL_MFMA_FP4 && kernel != CGH_KERNEL_MFMA_I8) {
+        set_err(err, errlen, "a one-hot in host memory needs an MFMA product kernel (got %d)", kernel);
+        return 1;
+    }
+    const int n = ds->n, ns = (n + S - 1) / S;
+    const size_t plane_bytes = ds->n_pad * 16;
+    const size_t valid = ((size_t)ds_kv1(ds) + 3) / 4;                    // planes that can hold a bit
+    const size_t cp = ds->chunk_planes, chunks = std::max<size_t>(1, (valid + cp - 1) / cp);
+    for (size_t c = 0; c < chunks; c++) {
+        const int k = (int)(c % 2);
+        const size_t p0 = c * cp, np = std::min(cp, ds->kw1 / 4 - p0);  // whole slabs (kw1 is)
+        const int kv1 = (int)std::min(np * 4, (size_t)ds_kv1(ds) - std::min((size_t)ds_kv1(ds), p0 * 4));
+        if (c >= 2) HIP_TRY(hipStreamWaitEvent(ds->ostream, ds->used[k], 0));
+        HIP_TRY(hipMemcpyAsync(ds->dOc[k], ds->hO + p0 * ds->n_pad * 4, np * plane_bytes, hipMemcpyHostToDevice,
+                               ds->ostream));
+        HIP_TRY(hipEventRecord(ds->loaded[k], ds->ostream));
+        HIP_TRY(hipStreamWaitEvent(st, ds->loaded[k], 0));
+        const bool first = c == 0, last = c + 1 == chunks;
+        HIP_TRY(launch_tiles(kernel, tile, ds->dOc[k], (int)(np * 4), ds->dP, first ? (int)ds->kw2 : 0,
+                             (int)ds->n_pad, n, 0, S, ns, d_out, st, ds->drstart, last ? ds->cap : 0, std::max(kv1, 1),
+                             first ? ds_kv2(ds) : 0, ds_stages(ds, kernel), first ? 0 : 1));
+        HIP_TRY(hipEventRecord(ds->used[k], st));
+    }
+    return 0;
+fail:
+    return 1;
+}
+
+}  // namespace
+
 extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* stream, int kernel, int phases,
                                  char* err, int errlen) {
     hipStream_t st = (hipStream_t)stream;
@@ -1689,10 +1852,13 @@ extern "C" int cgh_dataset_phase(cgh_dataset* ds, int S, uint16_t* d_out, void* 
         return 1;
     }
     HIP_TRY(hipSetDevice(ds->device));
-    if (phases & CGH_PHASE_PRODUCT)
+    if ((phases & CGH_PHASE_PRODUCT) && ds->hO) {
+        if (split_passes(ds, S, d_out, st, ds_kernel(ds, kernel), tile, err, errlen)) return 1;
+    } else if (phases & CGH_PHASE_PRODUCT) {
         HIP_TRY(launch_tiles(ds_kernel(ds, kernel), tile, ds->dO, (int)ds->kw1, ds->dP, (int)ds->kw2, (int)ds->n_pad, n, 0,
                              S, (n + S - 1) / S, d_out, st, ds->drstart, ds->cap, ds_kv1(ds), ds_kv2(ds),
                              ds_stages(ds, kernel)));
+    }
     if ((phases & CGH_PHASE_RARE) && ds->n_rare) {
         RareJob jb = rare_job(d_out, n, 0, S, ds->drstart, ds->drentry, ds->dgmem, ds->m16, ds->cap,
                               lanes_per_range(ds->n_rare, ds->n_rentries));
@@ -1834,6 +2000,10 @@ extern "C" int cgh_batch_create(int device, cgh_dataset* const* sets, int count,
     }
     for (int d = 0; d < count; d++) {
         const cgh_dataset* ds = sets[d];
+        if (split_unsupported(ds, "cgh_batch_create", err, errlen)) {
+            delete b;
+            return 1;
+        }
         if (ds->device != device) {
             set_err(err, errlen, "dataset %d lives on device %d, not %d", d, ds->device, device);
             delete b;
@@ -2196,6 +2366,8 @@ extern "C" int cgh_batch_run_noalloc(cgh_dataset* const* sets, int count, int S,
         set_err(err, errlen, "unknown kernel %d", kernel);
         return 1;
     }
+    for (int d = 0; d < count; d++)
+        if (split_unsupported(sets[d], "cgh_batch_run_noalloc", err, errlen)) return 1;
     if (((uintptr_t)d_out) & 15) {
         set_err(err, errlen, "output buffer must be 16-byte aligned");
         return 1;
@@ -2968,20 +3140,6 @@ double seconds_since(std::chrono::steady_clock::time_point t) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
 }
 
-// f(lo, hi) over [0, count) split into contiguous ranges on the host's
-// hardware threads.
-template <class F>
-void host_parallel(size_t count, F&& f) {
-    const size_t threads = std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), 64));
-    const size_t per = (count + threads - 1) / threads;
-    if (threads == 1 || count < 2 * threads) {
-        f((size_t)0, count);
-        return;
-    }
-    std::vector<std::thread> pool;
-    for (size_t lo = 0; lo < count; lo += per) pool.emplace_back([&f, lo, per, count] { f(lo, std::min(count, lo + per)); });
-    for (auto& t : pool) t.join();
-}
 
 // Device copy of a caller's rare CSR (start[n + 1], js[n_rare]: partners
 // j > i of row i, in any order), read by the row-local rare pass as one
@@ -3132,7 +3290,7 @@ struct DatasetStripes {
 // its callback run there.
 extern "C" int cgh_dataset_stream(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs, char* err,
                                   int errlen) {
-    if (!stripe_height_ok(S, err, errlen)) return 1;
+    if (!stripe_height_ok(S, err, errlen) || split_unsupported(ds, "cgh_dataset_stream", err, errlen)) return 1;
     const auto t_start = std::chrono::steady_clock::now();
     StripeRing ring;
     HIP_TRY(hipSetDevice(ds->device));
@@ -3151,7 +3309,7 @@ fail:
 // The same stripes as one captured HIP graph (StripeRing::run_graph).
 extern "C" int cgh_dataset_stream_graph(cgh_dataset* ds, int S, tch_cb cb, void* ctx, double* gpu_secs,
                                         double* build_ms, char* err, int errlen) {
-    if (!stripe_height_ok(S, err, errlen)) return 1;
+    if (!stripe_height_ok(S, err, errlen) || split_unsupported(ds, "cgh_dataset_stream_graph", err, errlen)) return 1;
     const auto t_start = std::chrono::steady_clock::now();
     StripeRing ring;
     HIP_TRY(hipSetDevice(ds->device));

@@ -144,6 +144,22 @@ typedef struct cgh_dataset cgh_dataset;
 /* Upload O, P and the rare groups of `enc` to `device`. */
 int cgh_dataset_upload(int device, const cgh_encoding* enc, cgh_dataset** out, char* err, int errlen);
 
+/* The same dataset with the one-hot O kept in pinned host memory and split
+ * along K: the product streams it through two device buffers of o_bytes / 2
+ * (whole 512-bit slabs of every row; at least one slab each), one pass per
+ * chunk, the upload of the next chunk overlapping the current pass.  Pass 0
+ * stores P P^T - O_0 O_0^T; each later pass subtracts O_c O_c^T from the
+ * stored u16 cells in place (every cell only falls towards the final value,
+ * so none underflows); the -X cap goes with the last pass, the rare pairs
+ * with the rare phase as usual.  The device then holds P, the rare groups,
+ * the two chunks and the output, not O.  cgh_dataset_upper and _phase run
+ * it with the MFMA kernels (CGH_KERNEL_VALU_POPC is refused); _stream,
+ * _stream_graph, _autotune and the batch refuse such a dataset.  Lifts the
+ * reference's limit that the one-hot fit in device memory
+ * (README.md:126-129); no counterpart in the reference. */
+int cgh_dataset_upload_split(int device, const cgh_encoding* enc, size_t o_bytes, cgh_dataset** out, char* err,
+                             int errlen);
+
 /* The same dataset from dense ranks X (n x L u16) at threshold tau, with the
  * O/P bits built on the device from X and the column table (no host bit
  * matrices: at 100,000 x 5,000 and tau = 1, O alone is 66 GB).  The rare
