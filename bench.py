@@ -429,6 +429,55 @@ def serve_record(args, cgh, wl, torch):
     }
 
 
+def split_record(args, cgh, wl, torch):
+    """The one-hot split along K (cgh_dataset_upload_split) on the 63k
+    long-tail set at tau 64 (a 0.68 GB one-hot of 86k columns): O stays in
+    pinned host memory and streams through device chunks of an eighth of it,
+    one product pass per chunk; timed against the in-core product over the
+    same operands, outputs compared cell for cell."""
+    import numpy as np
+
+    tau = 64
+    t = time.perf_counter()
+    X = cgh.dense_ranks(wl.long_tail(N, L, seed=2))
+    enc = cgh.Encoded(X, tau, 256)
+    del X
+    e = enc.enc
+    o_bytes_full = int(e.n_pad) * ((int(e.kw1) + 15) // 16 * 16) * 4
+    o_bytes = max(o_bytes_full // 8, 2 * 4 * int(e.n_pad) * 16)
+    log(f"split: tau={tau} D={int(e.D)} one-hot {o_bytes_full / 1e9:.2f} GB, device chunks {o_bytes / 1e9:.3f} GB "
+        f"(encoded in {time.perf_counter() - t:.1f}s)")
+    stream = torch.cuda.current_stream()
+    out = torch.empty(cgh.upper_elems(N, S), dtype=torch.int16, device="cuda")
+    res = {}
+    flats = []
+    for label, kw in (("in_core", {}), ("split", {"o_bytes": o_bytes})):
+        ds = cgh.Dataset(enc, 0, **kw)
+        ds.phase(S, out.data_ptr(), cgh.PHASE_PRODUCT, stream.cuda_stream)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record(stream)
+            ds.phase(S, out.data_ptr(), cgh.PHASE_PRODUCT, stream.cuda_stream)
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        flats.append(out.cpu())
+        ds.close()
+        res[label] = {"product_ms": float(np.median(ts))}
+    chunk_planes = o_bytes // 2 // (int(e.n_pad) * 16) // 4 * 4
+    planes = -(-int(e.D) // 128)
+    rec = {"workload": f"long_tail {N} x {L}, tau {tau}, D {int(e.D)}", "one_hot_gb": o_bytes_full / 1e9,
+           "device_chunk_gb": o_bytes / 1e9, "passes": -(-planes // chunk_planes),
+           "in_core": res["in_core"], "split": res["split"],
+           "slowdown": res["split"]["product_ms"] / res["in_core"]["product_ms"],
+           "identical_output": bool(torch.equal(flats[0], flats[1]))}
+    log(f"split: in-core product {res['in_core']['product_ms']:.1f} ms, split ({rec['passes']} passes) "
+        f"{res['split']['product_ms']:.1f} ms, identical output: {rec['identical_output']}")
+    return rec
+
+
 def native_oracle():
     """oracle/oracle.c compiled for this host's CPU (-march=native, OpenMP)
     into a temporary directory, loaded as the oracle's library."""
@@ -571,9 +620,9 @@ def main():
     ap.add_argument("--headline-only", action="store_true", help="skip the batch and serve records")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration) and exit")
-    ap.add_argument("--workload", default="all", choices=["all", "long_tail", "batch", "serve", "cli"],
+    ap.add_argument("--workload", default="all", choices=["all", "long_tail", "batch", "serve", "cli", "split"],
                     help="all: the headline line with its batch and serve records; long_tail: the headline "
-                         "alone; batch / serve / cli: that record alone")
+                         "alone; batch / serve / cli / split: that record alone")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
     ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="no GPU: turn four rocprofv3 --pmc csv directories into "
@@ -608,6 +657,9 @@ def main():
         return
     if args.workload == "cli":
         return run_cli(args, cgh, wl)
+    if args.workload == "split":
+        print(json.dumps(split_record(args, cgh, wl, torch)), flush=True)
+        return
 
     rec, X, flat = headline(args, cgh, wl, torch)
     if args.workload == "all" and not args.headline_only:

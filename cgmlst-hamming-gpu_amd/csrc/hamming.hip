@@ -335,10 +335,11 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
                 for (int j = 0; j < TN; j++)
                     nb[j] = reinterpret_cast<const uint2*>(&ring.b[buf][gi + 1][wn0 + j * 32 + l32])[g];
             }
-            // plane c * KG + gi (uniform): the first P plane negates acc,
-            // a plane past the end is skipped
+            // plane c * KG + gi (uniform): the first P plane negates acc (a
+            // pass with no P planes negates after the loop instead), a plane
+            // past the end is skipped
             const int p = c * KG + gi;
-            if (p == npO) {
+            if (p == npO && npP > 0) {
 #pragma unroll
                 for (int i = 0; i < TM; i++)
 #pragma unroll

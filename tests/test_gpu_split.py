@@ -21,13 +21,17 @@ def upper(cgh, gpu, ds, n, S, kernel):
 
 @pytest.mark.parametrize("kernel", [1, 2])
 @pytest.mark.parametrize("slabs", [1, 3, 1000])
-def test_split_one_hot_equals_in_core(cgh, gpu, workloads, kernel, slabs):
-    n, L, S, tau = 3000, 600, 512, 4
+@pytest.mark.parametrize("L", [600, 650])
+def test_split_one_hot_equals_in_core(cgh, gpu, workloads, kernel, slabs, L):
+    """L = 650 at 1-slab chunks ends with a pass holding one valid word, so
+    the plane after its last O plane falls inside the slab (the case that
+    caught an O-only pass negating its value twice)."""
+    n, S, tau = 3000, 512, 4
     X = cgh.dense_ranks(workloads.small_zipf(n, L, seed=21))
     enc = cgh.Encoded(X, tau, 256)
     n_pad = int(enc.enc.n_pad)
     o_bytes = 2 * slabs * 4 * n_pad * 16  # two buffers of `slabs` 512-bit slabs of every row
-    print(f"split one-hot kernel={kernel} slabs={slabs} D={int(enc.enc.D)}", flush=True)
+    print(f"split one-hot L={L} kernel={kernel} slabs={slabs} D={int(enc.enc.D)}", flush=True)
     ref_ds = cgh.Dataset(enc)
     want = upper(cgh, gpu, ref_ds, n, S, kernel)
     ref_ds.close()
