@@ -392,6 +392,58 @@ def test_rare_pass_many_chunks(cgh, gpu):
         assert np.array_equal(got, ref), i
 
 
+def upper_checksum(cgh, flat, n, S):
+    """tensorcore/tc_hamming.cu:241's checksum over the strict upper triangle
+    held in the stripe layout (row i's cells j > i are one contiguous run)."""
+    h = 0
+    for s, r0 in enumerate(range(0, n, S)):
+        off = cgh.stripe_offset(n, S, s)
+        w = n - r0
+        for r in range(min(S, n - r0)):
+            base = off + r * w
+            h = oracle.checksum_cells(flat[base + r + 1: base + w], h)
+    return h
+
+
+def test_rare_flat_walk_one_row_items(cgh, gpu, workloads):
+    """Rows longer than half an LDS chunk (n = 33,000 > 32,768): every rare
+    work item is one row, which takes the flat walk (interior member words
+    laid end to end over the batch, edge words per range), as on the
+    63,005-row benchmark set.  Long-tail allele counts give ranges from one
+    member to thousands.  The whole upper triangle, chained into
+    tc_hamming.cu:241's checksum, equals the oracle's at every tau (from
+    nearly all-frequent to all-rare) and both MFMA product forms; rows at the
+    stripe edges equal the oracle's rows, also under a cap."""
+    n, L, S = 33000, 40, 4096
+    X = cgh.dense_ranks(workloads.long_tail(n, L, seed=31))
+    print("flat walk: oracle checksum", flush=True)
+    want = oracle.checksum_upper(X)
+    rows = [0, 1, 4095, 4096, n // 2, n - 2]
+    ref = {i: oracle.brute_rows(X, i, i + 1)[0, i + 1:] for i in rows}
+    out = gpu.empty(cgh.upper_elems(n, S), dtype=gpu.int16, device="cuda")
+    st = gpu.cuda.current_stream()
+    for tau, kernel, cap in ((2, 2, 0), (64, 1, 0), (1024, 2, 0), (10**6, 2, 0), (64, 2, 17)):
+        print(f"flat walk: tau={tau} kernel={kernel} cap={cap}", flush=True)
+        enc = cgh.Encoded(X, tau, 256)
+        assert tau < 10**6 or enc.enc.D == 0
+        ds = cgh.Dataset(enc)
+        if cap:
+            ds.set_cap(cap)
+        out.fill_(0x6b6b)
+        ds.upper(S, out.data_ptr(), st.cuda_stream, kernel)
+        gpu.cuda.synchronize()
+        ds.close()
+        flat = out.cpu().numpy().view(np.uint16)
+        if not cap:
+            assert upper_checksum(cgh, flat, n, S) == want, (tau, kernel)
+        for i in rows:
+            s = i // S
+            r0 = s * S
+            base = cgh.stripe_offset(n, S, s) + (i - r0) * (n - r0)
+            got = flat[base + i + 1 - r0: base + n - r0]
+            assert np.array_equal(got, np.minimum(ref[i], cap) if cap else ref[i]), (tau, kernel, cap, i)
+
+
 @pytest.mark.parametrize("cap", [4464, 65535, 65536, 70000])
 def test_cap_at_and_beyond_u16(cgh, gpu, cap):
     """-X at and past the u16 range on rows that the rare pass finishes (tau
