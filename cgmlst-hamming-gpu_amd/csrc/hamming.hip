@@ -873,11 +873,13 @@ struct RareShared {
 // column chunk ch of it.  The cells are staged as one contiguous range
 // [g0, g1) of the layout (for several rows it includes their cells j <= i,
 // which no partner touches and which are written back unchanged), as nq
-// 16-byte words from a0.  False when the item has no rare partners.
+// 16-byte words from a0, into LDS words lb .. lb + nq - 1.  False when the
+// item has no rare partners.
 struct RareItem {
     int i0, i1, j0, j1;
     size_t sbase, sr0, g0, g1, a0;
     int nq;
+    int lb;  // first LDS word of the staged cells: 0, or RHALF (the upper half)
 };
 
 __device__ __forceinline__ bool rare_item(const RareJob& jb, long long b, RareItem& it) {
@@ -912,6 +914,7 @@ __device__ __forceinline__ bool rare_item(const RareJob& jb, long long b, RareIt
     it.g1 = it.i1 - it.i0 == 1 ? row_base(it.i0) + it.j1 : row_base(it.i1 - 1) + jhi;
     it.a0 = it.g0 & ~(size_t)7;                    // first 16-byte word touched
     it.nq = (int)((it.g1 - it.a0 + 7) >> 3);       // 16-byte words staged
+    it.lb = 0;
     return true;
 }
 
@@ -935,9 +938,35 @@ __device__ __forceinline__ void rare_stage_issue(const RareJob& jb, const RareIt
 template <size_t... U>
 __device__ __forceinline__ void rare_stage_land(const RareItem& it, const v4i (&v)[RQ], RareShared& sh,
                                                 std::index_sequence<U...>) {
-    v4i* cells = reinterpret_cast<v4i*>(sh.cells4);
+    v4i* cells = reinterpret_cast<v4i*>(sh.cells4 + it.lb);
     const int t = threadIdx.x;
     ((t + (int)U * RNT < it.nq ? (void)(cells[t + U * RNT] = v[U]) : (void)0), ...);
+}
+
+// Items of at most RHALF words are staged in one half of the LDS cells
+// (words 0 .. RHALF-1, or RHALF .. 2 RHALF; the array holds 2 RHALF + 1), so
+// the next such item can land in the other half while one is walked.
+constexpr int RHALF = RCW / 16;
+
+// An item's nq words straight from global memory into LDS words it.lb ..
+// (global_load_lds_dwordx4: lane l of a wavefront lands at the wavefront's
+// LDS base + 16 l), 64 words per wavefront instruction; lanes past the item
+// reread its last word into the half's spare words (it.nq <= RHALF keeps all
+// of them inside the half).  The loads count in the issuing wavefront's
+// vmcnt: the caller waits for them (vmcnt 0) and then for every wavefront (a
+// barrier) before the cells are used.  Written as an instruction rather than
+// the builtin so that the compiler does not wait for these transfers before
+// every LDS access of the walk running meanwhile in the other half.
+__device__ __forceinline__ void rare_stage_dma(const RareJob& jb, const RareItem& it, RareShared& sh) {
+    const v4i* __restrict__ src = reinterpret_cast<const v4i*>(jb.out + it.a0);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int blocks = (it.nq + 63) / 64;
+    for (int q = wave; q < blocks; q += RNT / 64) {
+        const v4i* p = src + min(q * 64 + lane, it.nq - 1);
+        const uint32_t lds = __builtin_amdgcn_readfirstlane(
+            (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)&sh.cells4[it.lb + q * 64]);
+        asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(p), "s"(lds) : "memory", "m0");
+    }
 }
 
 // Every rare partner of the item's rows is subtracted from its staged cell.
@@ -1009,7 +1038,7 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
             const int rsg = jb.rsg, sub = tid / rsg, sl = tid % rsg, leader = (tid & 63) - sl;
             char* const lds = reinterpret_cast<char*>(sh.cells4);
             const uint32_t own = (uint32_t)(reinterpret_cast<char*>(&sh.spare[tid & 63]) - lds);
-            const uint32_t bo = (uint32_t)(base - it.a0);  // LDS u16 offset of cell (i, 0), mod 2^32
+            const uint32_t bo = (uint32_t)(base - it.a0) + 8u * it.lb;  // LDS u16 offset of cell (i, 0), mod 2^32
             const uint32_t ustep = (uint32_t)(rsg * WPL);         // words per unit
             auto load_unit = [&](v4i (&w)[WPL], uint32_t uw, uint32_t end) {
                 const uint32_t last = (max(end, 1u) - 1) / PER;
@@ -1135,7 +1164,8 @@ __device__ __forceinline__ void rare_ranges_flat(const RareJob& jb, const M* __r
     };
     for (int i = it.i0; i < it.i1; i++) {
         const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
-        const uint32_t bo = (uint32_t)(rare_row_base(jb, it, i) - it.a0);  // LDS u16 offset of cell (i, 0)
+        // LDS u16 offset of cell (i, 0), mod 2^32
+        const uint32_t bo = (uint32_t)(rare_row_base(jb, it, i) - it.a0) + 8u * it.lb;
         for (uint32_t eb = e0; eb < e1; eb += RNT) {
             const int nb = (int)min((uint32_t)RNT, e1 - eb);
             uint32_t b = 0, end = 0;
@@ -1320,8 +1350,9 @@ __device__ __forceinline__ void rare_ranges_flat(const RareJob& jb, const M* __r
 // cell by cell (their other cells belong to other items).
 __device__ __forceinline__ void rare_write_back(const RareJob& jb, const RareItem& it, RareShared& sh) {
     const int tid = threadIdx.x;
+    const uint4* cells = sh.cells4 + it.lb;
     if (jb.cap > 0) {  // main.rs:251-253 on the final values
-        uint16_t* c16 = reinterpret_cast<uint16_t*>(sh.cells4);
+        uint16_t* c16 = reinterpret_cast<uint16_t*>(sh.cells4 + it.lb);
         const uint16_t cap = (uint16_t)min(jb.cap, 0xffff);
         for (int k = tid; k < it.nq * 8; k += RNT) c16[k] = min(c16[k], cap);
         __syncthreads();
@@ -1331,9 +1362,9 @@ __device__ __forceinline__ void rare_write_back(const RareJob& jb, const RareIte
     for (int k = tid; k < it.nq; k += RNT) {
         const size_t lo = it.a0 + 8 * (size_t)k;  // u16 index of the word's first cell
         if (lo >= it.g0 && lo + 8 <= it.g1) {
-            dst[k] = sh.cells4[k];
+            dst[k] = cells[k];
         } else {
-            const uint16_t* c16 = reinterpret_cast<const uint16_t*>(&sh.cells4[k]);
+            const uint16_t* c16 = reinterpret_cast<const uint16_t*>(&cells[k]);
             for (int h = 0; h < 8; h++)
                 if (lo + h >= it.g0 && lo + h < it.g1) out[lo + h] = c16[h];
         }
@@ -1341,15 +1372,21 @@ __device__ __forceinline__ void rare_write_back(const RareJob& jb, const RareIte
 }
 
 // Items b0, b0 + step, ... < nitems of job jb, one after the other in this
-// workgroup.  The next item's staging loads are issued before the current
-// item is written back, so their latency is spent under the write-back
-// instead of in front of the next item's atomics.
-// (Measured and not kept: issuing them, and the next item's first partner
-// entries, as soon as the current item has landed: the registers they hold
-// through the walk cost spills, 4.9 against 4.7 ms; counting partners in a
-// cleared LDS buffer and subtracting the counts from the output with
-// device-scope atomics instead of staging: 11.7 ms, the L2's atomic rate
-// over 4 GB of output words.)
+// workgroup.
+// When this item and the next both fit half of the LDS cells (at most RHALF
+// words), the next item's cells are copied straight into the other half
+// before this item's walk starts (rare_stage_dma): the walk's first load wait
+// covers them (loads complete in order), and the next item starts with its
+// cells in place, with no staging round trip of its own.
+// Otherwise the next item's staging loads go to registers before this item
+// is written back, so their latency is spent under the write-back, and the
+// next item lands at word 0 after it.
+// (Measured and not kept: issuing the register staging loads, and the next
+// item's first partner entries, as soon as the current item has landed: the
+// registers they hold through the walk cost spills, 4.9 against 4.7 ms;
+// counting partners in a cleared LDS buffer and subtracting the counts from
+// the output with device-scope atomics instead of staging: 11.7 ms, the L2's
+// atomic rate over 4 GB of output words.)
 template <typename M, bool MASK>
 __device__ __forceinline__ void rare_items(const RareJob& jb, const M* __restrict__ gmem, long long b0,
                                            long long nitems, long long step, RareShared& sh) {
@@ -1360,8 +1397,17 @@ __device__ __forceinline__ void rare_items(const RareJob& jb, const M* __restric
     constexpr auto SEQ = std::make_index_sequence<RQ>();
     v4i v[RQ];
     rare_stage_issue(jb, cur, v, SEQ);
+    bool landed = false;  // cur's cells are in LDS already (copied during the previous walk)
     while (true) {
-        rare_stage_land(cur, v, sh, SEQ);
+        if (!landed) rare_stage_land(cur, v, sh, SEQ);
+        long long nb = b + step;
+        while (nb < nitems && !rare_item(jb, nb, nxt)) nb += step;
+        const bool more = nb < nitems;
+        const bool dma = more && cur.nq <= RHALF && nxt.nq <= RHALF;  // uniform
+        if (dma) {
+            nxt.lb = cur.lb ? 0 : RHALF;
+            rare_stage_dma(jb, nxt, sh);
+        }
         // Ascending ranges: one-row items take the flat walk, items of several
         // short rows the subgroup walk, whose fixed cost per row is lower (the
         // flat walk on those made a batch of 64 sets of 500 to 20,000 rows
@@ -1370,12 +1416,12 @@ __device__ __forceinline__ void rare_items(const RareJob& jb, const M* __restric
         if constexpr (MASK) rare_ranges<M, true>(jb, gmem, cur, sh);
         else if (cur.i1 - cur.i0 == 1) rare_ranges_flat<M>(jb, gmem, cur, sh);
         else rare_ranges<M, false>(jb, gmem, cur, sh);
-        long long nb = b + step;
-        while (nb < nitems && !rare_item(jb, nb, nxt)) nb += step;
-        if (nb < nitems) rare_stage_issue(jb, nxt, v, SEQ);
+        if (more && !dma) rare_stage_issue(jb, nxt, v, SEQ);  // nxt.lb == 0
         rare_write_back(jb, cur, sh);
-        if (nb >= nitems) return;
-        __syncthreads();  // the write-back's LDS reads before the next item lands
+        if (!more) return;
+        if (dma) __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wavefront's copies have landed
+        __syncthreads();  // the write-back's LDS reads before the next item lands; every copy landed
+        landed = dma;
         b = nb;
         cur = nxt;
     }
