@@ -258,7 +258,20 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
     // of the wavefront lands at the wavefront's LDS base + 16 l).  Rows past
     // n read row n-1: their products only reach output cells that are never
     // stored.
-    auto issue_slab = [&](int c, int st) {
+    // What does not change from slab to slab is set up once per tile: for
+    // each of the wavefront's loads u, its 4-word group, the O and P bases of
+    // its side, the bytes per plane, the lane's row offset in a plane and the
+    // LDS address of its 64-row block in stage 0.  A slab's load then costs a
+    // plane index, a base select and a 64-bit add in scalar registers (the
+    // bases selected per slab from the kernel arguments were reloaded from
+    // them every time, with a scalar-memory wait each).
+    const char* base_o[LPW];
+    const char* base_p[LPW];
+    uint32_t plane_bytes[LPW], row_off[LPW], lds_blk[LPW], lds_stage[LPW];
+    int grp_of[LPW];
+    {
+        using Lds = __attribute__((address_space(3))) char;
+        const uint32_t lds_ring = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(Lds*)&ring);
 #pragma unroll
         for (int u = 0; u < LPW; u++) {
             const int v = wave * LPW + u;
@@ -266,28 +279,43 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
             const int rem = side ? v - KG * (BM / 64) : v;
             const int nrb = side ? BN / 64 : BM / 64;
             const int grp = rem / nrb, rb = rem % nrb;
+            grp_of[u] = grp;
+            base_o[u] = reinterpret_cast<const char*>(side ? a.OB : a.O);
+            base_p[u] = reinterpret_cast<const char*>(side ? a.PB : a.P);
+            plane_bytes[u] = (uint32_t)(side ? a.ldB : a.ldA) * 16u;
+            const int row = side ? min(growB + rb * 64 + lane, a.nB - 1) : min(growA + rb * 64 + lane, a.n - 1);
+            row_off[u] = (uint32_t)row * 16u;
+            // ring.a[st][grp][r] / ring.b[st][grp][r] as byte offsets from the ring
+            lds_blk[u] = lds_ring + (side ? (uint32_t)(NST * KG * BM * 16 + (grp * BN + rb * 64) * 16)
+                                          : (uint32_t)((grp * BM + rb * 64) * 16));
+            lds_stage[u] = side ? KG * BN * 16 : KG * BM * 16;
+        }
+    }
+    auto issue_slab = [&](int c, int st) {
+#pragma unroll
+        for (int u = 0; u < LPW; u++) {
             bool isO;
             int plane;
             if (FP4) {
-                const int p = c * KG + grp;
+                const int p = c * KG + grp_of[u];
                 isO = p < npO || npP == 0;  // with no P planes a plane past the end reads O plane 0
                 plane = p < npO ? p : npP ? min(p - npO, npP - 1) : 0;
             } else {
                 isO = c < nc1;
-                plane = (isO ? c : c - nc1) * KG + grp;
+                plane = (isO ? c : c - nc1) * KG + grp_of[u];
             }
-            const uint32_t* base = side ? (isO ? a.OB : a.PB) : (isO ? a.O : a.P);
-            const int grow = side ? min(growB + rb * 64 + lane, a.nB - 1) : min(growA + rb * 64 + lane, a.n - 1);
-            const uint32_t* src = base + ((size_t)plane * (size_t)(side ? a.ldB : a.ldA) + grow) * 4;
-            // LDS base of this wavefront's 64 rows goes to M0.  Written as an
-            // instruction rather than the builtin so that the compiler does
-            // not wait for every LDS-DMA transfer before each LDS read (it
-            // cannot tell the ring stages apart); the waits are explicit
-            // below.  The load has no register result.
-            const uint32_t lds = __builtin_amdgcn_readfirstlane(
-                (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)(side ? &ring.b[st][grp][rb * 64]
-                                                                                     : &ring.a[st][grp][rb * 64]));
-            asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(src), "s"(lds) : "memory", "m0");
+            const char* base = (isO ? base_o[u] : base_p[u]) + (size_t)plane * plane_bytes[u];
+            const uint32_t lds = lds_blk[u] + (uint32_t)st * lds_stage[u];
+            // The LDS base of this wavefront's 64 rows goes to M0; the global
+            // address is the scalar base plus the lane's 32-bit row offset.
+            // Written as an instruction rather than the builtin so that the
+            // compiler does not wait for every LDS-DMA transfer before each
+            // LDS read (it cannot tell the ring stages apart); the waits are
+            // explicit below.  The load has no register result.
+            asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1"
+                         :
+                         : "v"(row_off[u]), "s"(base), "s"(lds)
+                         : "memory", "m0");
         }
     };
     // Every step issues exactly one slab (past the last slab it reloads slab
