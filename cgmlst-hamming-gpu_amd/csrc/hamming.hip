@@ -291,20 +291,40 @@ __device__ __forceinline__ void tile_body(const TileJob& a, int s, int I, int J,
             lds_stage[u] = side ? KG * BN * 16 : KG * BM * 16;
         }
     }
+    // The global address of load u for slab c.
+    auto slab_addr = [&](int c, int u) -> const char* {
+        bool isO;
+        int plane;
+        if (FP4) {
+            const int p = c * KG + grp_of[u];
+            isO = p < npO || npP == 0;  // with no P planes a plane past the end reads O plane 0
+            plane = p < npO ? p : npP ? min(p - npO, npP - 1) : 0;
+        } else {
+            isO = c < nc1;
+            plane = (isO ? c : c - nc1) * KG + grp_of[u];
+        }
+        return (isO ? base_o[u] : base_p[u]) + (size_t)plane * plane_bytes[u];
+    };
+    // Running addresses of the last slab issued (cur): inside the O planes
+    // and inside the P planes, slab c + 1 of a load is slab c plus KG planes,
+    // so only the slabs where some load crosses from O to P or reaches the
+    // end (and the first slab) recompute them.
+    const char* addr[LPW];
+    int cur = -2;
     auto issue_slab = [&](int c, int st) {
+        const bool step1 = c == cur + 1 &&
+                           (FP4 ? !((c * KG <= npO + KG && (c + 1) * KG >= npO) || (c + 1) * KG > npl) : c != nc1);
+        if (step1) {
+#pragma unroll
+            for (int u = 0; u < LPW; u++) addr[u] += (size_t)KG * plane_bytes[u];
+        } else if (c != cur) {
+#pragma unroll
+            for (int u = 0; u < LPW; u++) addr[u] = slab_addr(c, u);
+        }
+        cur = c;
 #pragma unroll
         for (int u = 0; u < LPW; u++) {
-            bool isO;
-            int plane;
-            if (FP4) {
-                const int p = c * KG + grp_of[u];
-                isO = p < npO || npP == 0;  // with no P planes a plane past the end reads O plane 0
-                plane = p < npO ? p : npP ? min(p - npO, npP - 1) : 0;
-            } else {
-                isO = c < nc1;
-                plane = (isO ? c : c - nc1) * KG + grp_of[u];
-            }
-            const char* base = (isO ? base_o[u] : base_p[u]) + (size_t)plane * plane_bytes[u];
+            const char* base = addr[u];
             const uint32_t lds = lds_blk[u] + (uint32_t)st * lds_stage[u];
             // The LDS base of this wavefront's 64 rows goes to M0; the global
             // address is the scalar base plus the lane's 32-bit row offset.
