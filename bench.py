@@ -200,8 +200,10 @@ def headline(args, cgh, wl, torch):
                          f"({members} members, {entries} ranges); the {R} member reads (2 B each) hit the caches",
                  "ms": rare_ms, "frac_of_measured_copy": rare_gbs / memory["copy_gbs"],
                  "rare_pairs_per_s": R / (rare_ms * 1e-3),
-                 "limiter": "VALU issue: ~8 vector instructions per rare pair (LDS address, half-word value, "
-                            "range mask) on 16 waves per CU; see profiles/sub_rare_rows_counters_*.md"},
+                 "limiter": "per-item chain of memory round trips (cell staging, range entries and edge "
+                            "words, then the walk's first chunk), which every wave of the CU waits for at "
+                            "the item barriers; ~11 vector instructions per rare pair; see "
+                            "profiles/sub_rare_rows_counters_*.md"},
     }
     if traffic["rare"]:  # L2 <-> fabric bytes the counters saw, against the cache-resident read rate
         rooflines["rare"]["l2_fabric"] = {"achieved": traffic["rare"] / (rare_ms * 1e-3) / 1e9,
