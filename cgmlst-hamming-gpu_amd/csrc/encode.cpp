@@ -18,6 +18,8 @@
 #include <atomic>
 #include <vector>
 
+#include <omp.h>
+
 #include "cgmlst_hamming.h"
 
 namespace {
@@ -389,10 +391,25 @@ extern "C" int cgh_choose_tau(const uint32_t* cnt, size_t ncnt, int n, int L, si
 // Query codec of a resident database
 // ---------------------------------------------------------------------------
 
+// Slot of id v in a table of 2^(64 - shift) slots (Fibonacci hashing: the
+// top bits of v times 2^64 / golden ratio).
+static inline size_t codec_hash(uint64_t v, int shift) { return (size_t)((v * 0x9E3779B97F4A7C15ull) >> shift); }
+
 struct cgh_codec {
     int n = 0, L = 0, tau = 0;
     size_t kw1 = 0, kw2 = 0, D = 0;
-    std::vector<std::vector<uint64_t>> ids;  // locus -> sorted distinct ids; rank a <-> ids[k][a-1]
+    // locus -> open-addressing table of the database's distinct ids:
+    // 2^(64 - shift[k]) slots from slot off[k] on, at most half of them used,
+    // id 0 (never a call) marking an empty slot.  A slot holds what a query
+    // call with that id adds: its one-hot column x (y = ~0), or the member
+    // range [x, y) of its rare list.
+    struct Slot {
+        uint64_t id;
+        uint32_t x, y;
+    };
+    std::vector<Slot> slots;
+    std::vector<size_t> off;
+    std::vector<int> shift;
     std::vector<size_t> base;                // locus -> first one-hot column
     std::vector<std::vector<int32_t>> col;   // locus, rank -> local frequent column, or -1
     std::vector<std::vector<int64_t>> lst;   // locus, rank -> rare list number, or -1
@@ -415,7 +432,7 @@ extern "C" int cgh_codec_create(const uint64_t* calls, int n, int L, int tau, cg
     c->n = n;
     c->L = L;
     c->tau = tau = std::min(tau, n + 1);
-    c->ids.resize((size_t)L);
+    std::vector<std::vector<uint64_t>> ids((size_t)L);  // locus -> sorted distinct ids; rank a <-> ids[k][a-1]
     c->col.resize((size_t)L);
     c->lst.resize((size_t)L);
     std::vector<std::vector<uint32_t>> cnt;
@@ -423,7 +440,7 @@ extern "C" int cgh_codec_create(const uint64_t* calls, int n, int L, int tau, cg
     std::vector<size_t> nf((size_t)L, 0), nl((size_t)L, 0), nm((size_t)L, 0);
 #pragma omp parallel for schedule(dynamic, 4)
     for (int k = 0; k < L; k++) {
-        std::vector<uint64_t>& u = c->ids[k];
+        std::vector<uint64_t>& u = ids[k];
         u.reserve(cnt[k].size() - 1);
         for (int i = 0; i < n; i++)
             if (calls[(size_t)i * L + k]) u.push_back(calls[(size_t)i * L + k]);
@@ -454,6 +471,11 @@ extern "C" int cgh_codec_create(const uint64_t* calls, int n, int L, int tau, cg
         set_err(err, errlen, "database rare lists exceed 2^32 members: raise tau");
         return 1;
     }
+    if (c->base[L] >= 0xffffffffull) {  // columns are kept as u32 in the id tables
+        delete c;
+        set_err(err, errlen, "database one-hot exceeds 2^32 columns: raise tau");
+        return 1;
+    }
     c->D = c->base[L];
     c->kw1 = up(std::max<size_t>(c->D, 1), 256) / 32;
     c->kw2 = up((size_t)L, 256) / 32;
@@ -481,6 +503,30 @@ extern "C" int cgh_codec_create(const uint64_t* calls, int n, int L, int tau, cg
         }
     }
     c->lstart[l0[L]] = (uint32_t)m0[L];
+    c->off.assign((size_t)L + 1, 0);
+    c->shift.assign((size_t)L, 63);
+    for (int k = 0; k < L; k++) {
+        int bits = 1;
+        while (((size_t)1 << bits) < 2 * ids[k].size()) bits++;
+        c->shift[k] = 64 - bits;
+        c->off[k + 1] = c->off[k] + ((size_t)1 << bits);
+    }
+    c->slots.assign(c->off[L], cgh_codec::Slot{0, 0, 0});
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int k = 0; k < L; k++) {
+        cgh_codec::Slot* t = c->slots.data() + c->off[k];
+        const size_t mask = c->off[k + 1] - c->off[k] - 1;
+        for (size_t a = 1; a <= ids[k].size(); a++) {
+            size_t h = codec_hash(ids[k][a - 1], c->shift[k]);
+            while (t[h].id) h = (h + 1) & mask;
+            if (c->col[k][a] >= 0) {
+                t[h] = cgh_codec::Slot{ids[k][a - 1], (uint32_t)(c->base[k] + (size_t)c->col[k][a]), ~0u};
+            } else {
+                const size_t g = (size_t)c->lst[k][a];
+                t[h] = cgh_codec::Slot{ids[k][a - 1], c->lstart[g], c->lstart[g + 1]};
+            }
+        }
+    }
     if (enc) {
         if (cgh_encode(X.data(), n, L, tau, 256, enc, err, errlen)) {
             delete c;
@@ -518,35 +564,42 @@ extern "C" long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int 
                 (size_t)q * (size_t)L, cap);
         return -1;
     }
-    // row r writes its entries from slot r * L on; compacted below
+    // Row r writes its entries from slot r * L on; compacted below.  The
+    // queries go in blocks of qb, locus by locus within a block, so that a
+    // locus's id table is probed qb times while it is in cache; blocks are
+    // spread over the threads, qb chosen so that every thread gets one.
     std::atomic<int> bad{0};
-#pragma omp parallel for schedule(static)
-    for (int r = 0; r < q; r++) {
-        uint32_t* orow = O + (size_t)r * c->kw1;
-        uint32_t* prow = P + (size_t)r * c->kw2;
-        memset(orow, 0, c->kw1 * 4);
-        memset(prow, 0, c->kw2 * 4);
-        uint64_t* re = rentry + (size_t)r * L;
-        uint32_t m = 0;
+    const int qb = std::max(1, std::min(16, q / std::max(1, omp_get_max_threads())));
+    const int nblk = (q + qb - 1) / qb;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int blk = 0; blk < nblk; blk++) {
+        const int r0 = blk * qb, r1 = std::min(q, r0 + qb);
+        uint32_t m[16] = {};
+        for (int r = r0; r < r1; r++) {
+            memset(O + (size_t)r * c->kw1, 0, c->kw1 * 4);
+            memset(P + (size_t)r * c->kw2, 0, c->kw2 * 4);
+        }
         for (int k = 0; k < L; k++) {
-            const uint64_t v = calls[(size_t)r * L + k];
-            if (!v) continue;
-            prow[k >> 5] |= 1u << (k & 31);
-            const std::vector<uint64_t>& u = c->ids[k];
-            const auto it = std::lower_bound(u.begin(), u.end(), v);
-            if (it == u.end() || *it != v) continue;  // an id the database never saw matches no row
-            const size_t a = (size_t)(it - u.begin()) + 1;
-            if (c->col[k][a] >= 0) {
-                const size_t cc = c->base[k] + (size_t)c->col[k][a];
-                orow[cc >> 5] |= 1u << (cc & 31);
-            } else {
-                const size_t g = (size_t)c->lst[k][a];
-                const uint32_t b = c->lstart[g], e = c->lstart[g + 1];
-                if (b > e || e > c->members.size()) bad = 1;  // consumers read members[b, e)
-                re[m++] = ((uint64_t)e << 32) | b;
+            const cgh_codec::Slot* t = c->slots.data() + c->off[k];
+            const size_t mask = c->off[k + 1] - c->off[k] - 1;
+            const int sh = c->shift[k];
+            for (int r = r0; r < r1; r++) {
+                const uint64_t v = calls[(size_t)r * L + k];
+                if (!v) continue;
+                P[(size_t)r * c->kw2 + (k >> 5)] |= 1u << (k & 31);
+                size_t h = codec_hash(v, sh);
+                while (t[h].id && t[h].id != v) h = (h + 1) & mask;
+                if (!t[h].id) continue;  // an id the database never saw matches no row
+                const uint32_t x = t[h].x, y = t[h].y;
+                if (y == ~0u) {
+                    O[(size_t)r * c->kw1 + (x >> 5)] |= 1u << (x & 31);
+                } else {
+                    if (x > y || y > c->members.size()) bad = 1;  // consumers read members[x, y)
+                    rentry[(size_t)r * L + m[r - r0]++] = ((uint64_t)y << 32) | x;
+                }
             }
         }
-        rstart[r + 1] = m;  // count for now
+        for (int r = r0; r < r1; r++) rstart[r + 1] = m[r - r0];  // counts for now
     }
     if (bad) {
         set_err(err, errlen, "cgh_codec_encode: rare list outside the member array");
