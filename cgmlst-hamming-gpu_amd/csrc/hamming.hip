@@ -1033,13 +1033,20 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
     uint32_t* ebeg = sh.ebeg;
     uint32_t* eend = sh.eend;
     const int tid = threadIdx.x;
+    // The rows' entries are contiguous in rentry, so the batch after the one
+    // at eb of row i starts at eb + RNT inside the row, else at the row's end
+    // (the start of the next row with entries).  Thread tid's entry of the
+    // next batch is loaded while the current batch is walked.
+    const uint32_t E1 = rstart[it.i1];
+    uint64_t rn = 0;
+    if (rstart[it.i0] + tid < E1) rn = rentry[rstart[it.i0] + tid];
     for (int i = it.i0; i < it.i1; i++) {
         const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
         const size_t base = rare_row_base(jb, it, i);
         for (uint32_t eb = e0; eb < e1; eb += RNT) {
             const int nb = (int)min((uint32_t)RNT, e1 - eb);
             if (tid < nb) {
-                const uint64_t r = rentry[eb + tid];
+                const uint64_t r = rn;
                 uint32_t b = (uint32_t)r, end = (uint32_t)(r >> 32);
                 if (!MASK && jb.chunks > 1) {  // clip the ascending range to [j0, j1)
                     uint32_t lo = b, hi = end;
@@ -1060,6 +1067,10 @@ __device__ __forceinline__ void rare_ranges(const RareJob& jb, const M* __restri
             }
             if (tid == 0) sh.next = RNT / jb.rsg;  // ranges below this are taken without asking
             __syncthreads();
+            {
+                const uint32_t nbs = eb + RNT < e1 ? eb + RNT : e1;
+                rn = nbs + tid < E1 ? rentry[nbs + tid] : 0;
+            }
             // Subgroup sub starts with range sub and then takes the next free
             // one from sh.next, so a few long ranges do not leave the other
             // subgroups idle at the barrier.  A range is walked in units of
