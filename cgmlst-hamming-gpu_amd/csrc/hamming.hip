@@ -855,14 +855,16 @@ constexpr int RCW = 65536;  // u16 cells per chunk (128 KiB of LDS: one workgrou
 constexpr int RNT = 1024;   // threads per workgroup = ranges per batch
 constexpr int RQ = (RCW / 8 + 1 + RNT - 1) / RNT;  // 16-byte words staged per thread
 constexpr int MPL = 8;      // members per lane per unit of a range walk (one 16-byte word of u16 members)
-// Lanes per range: about 0.4 x the mean range length (partners per range),
-// a power of two in [4, 16]: short ranges leave fewer lanes idle, long ones
-// keep fewer ranges in flight.
+// Lanes per range of the subgroup walk from the mean range length (partners
+// per range): 4 below 20, 8 below 160, else 16.  Short ranges leave fewer
+// lanes idle, long ones keep fewer ranges in flight.  Measured rare pass:
+// mean 41 (12,000 rows, tau 256), 0.27 ms with 8 lanes against 0.32 ms with
+// 16; mean 286 (30,000 rows, tau 2048), 2.57 ms with 16 against 2.66 ms
+// with 8; the 64-set batch (means 4 to 170) 49.7 ms against 52.9 ms for the
+// rule that gave 16 from a mean of 40 on.
 int lanes_per_range(uint64_t pairs, size_t ranges) {
     const double mean = ranges ? (double)pairs / (double)ranges : 1.0;
-    int g = 4;
-    while (g < 16 && g * 2 <= mean * 0.4) g *= 2;
-    return g;
+    return mean >= 160.0 ? 16 : mean >= 20.0 ? 8 : 4;
 }
 struct RareJob {
     uint16_t* out;             // stripe layout of rows r0..
