@@ -855,6 +855,7 @@ constexpr int RCW = 65536;  // u16 cells per chunk (128 KiB of LDS: one workgrou
 constexpr int RNT = 1024;   // threads per workgroup = ranges per batch
 constexpr int RQ = (RCW / 8 + 1 + RNT - 1) / RNT;  // 16-byte words staged per thread
 constexpr int MPL = 8;      // members per lane per unit of a range walk (one 16-byte word of u16 members)
+constexpr int RFG = 1024;   // most rows of an item the flat walk takes (items of more take the subgroup walk)
 // Lanes per range of the subgroup walk from the mean range length (partners
 // per range): 4 below 20, 8 below 160, else 16.  Short ranges leave fewer
 // lanes idle, long ones keep fewer ranges in flight.  Measured rare pass:
@@ -880,6 +881,13 @@ struct RareJob {
     // G = 1: one row per workgroup, in `chunks` column chunks.
     int G, gps;
     int rsg;                   // lanes per range (a power of two, 4 .. 32)
+    // Items of several rows take the flat walk rather than the subgroup walk
+    // (set for long ranges, 16 lanes or more: a mean range length of 160 on).
+    // Measured rare pass: 30,000 rows at tau 2048 (mean 286), 2.33 ms against
+    // 2.55 ms; at a mean of 41 (12,000 rows, tau 256) 0.38 against 0.27 ms
+    // and at 12 (4,000 rows, tau 64) 0.24 against 0.06 ms, where its two
+    // edge words per range outweigh the short interiors.
+    int mflat;
     // Members ascending within each range (group member lists).  A range
     // in arbitrary order (a caller's per-row partner list) is walked whole in
     // every column chunk, members outside the chunk masked out.
@@ -902,6 +910,7 @@ RareJob rare_job(uint16_t* out, int n, int r0, int S, const uint32_t* rstart, co
     j.cap = cap;
     j.G = 1;
     j.rsg = rsg;
+    j.mflat = rsg >= 16;
     j.sorted = 1;
     return j;
 }
@@ -916,6 +925,10 @@ struct RareShared {
     uint32_t spare[64];  // one word per lane for the atomics of idle slots
     uint32_t wsum[RNT / 64];  // interior words per wavefront's ranges (scan)
     uint32_t next;       // next range of the batch to take
+    // Flat walk of an item of several rows: LDS u16 offset of cell (i, 0)
+    // of each batch range's row i, and the item's row starts in rentry.
+    uint32_t wbo[RNT];
+    uint32_t rs[RFG + 1];
 };
 
 // Work item b of job jb: with row groups (G > 1) rows i0 .. i1-1 of one
@@ -1191,7 +1204,12 @@ __device__ __forceinline__ void rare_dec(char* lds, uint32_t o) {
 // starts that fall inside the chunk.  Every member of an interior word is a
 // partner: no per-member test.  The thread of range r applies its edge
 // words' members itself, with a test per member.
-template <typename M>
+// MR: an item of several rows (at most RFG).  Their entries are contiguous in
+// rentry, so the batches run over all of them at once; each range's row is
+// found by a binary search of the rows' starts (staged in LDS), its cell
+// offset kept in wbo, and the walk carries each loaded word's offset with
+// it through the register ring.
+template <typename M, bool MR>
 __device__ __forceinline__ void rare_ranges_flat(const RareJob& jb, const M* __restrict__ gmem, const RareItem& it,
                                                  RareShared& sh) {
     constexpr int PER = 16 / (int)sizeof(M);  // members per 16-byte word
@@ -1223,8 +1241,14 @@ __device__ __forceinline__ void rare_ranges_flat(const RareJob& jb, const M* __r
             rare_dec(lds, bo + (PER == 8 ? __builtin_amdgcn_ubfe(xv, (h & 1) * 16, 16) : xv));
         }
     };
-    for (int i = it.i0; i < it.i1; i++) {
-        const uint32_t e0 = rstart[i], e1 = rstart[i + 1];
+    const uint32_t pitch = (uint32_t)((size_t)jb.n - it.sr0);  // MR: cell (i + 1, 0) - cell (i, 0)
+    if constexpr (MR) {  // visible after the first batch's scan barrier, before any search
+        int k = tid;
+        asm volatile("" : "+v"(k));  // (formed here: a value kept from the kernel's start was spilled)
+        for (; k <= it.i1 - it.i0; k += RNT) sh.rs[k] = rstart[it.i0 + k];
+    }
+    for (int i = it.i0; i < (MR ? it.i0 + 1 : it.i1); i++) {
+        const uint32_t e0 = rstart[i], e1 = rstart[MR ? it.i1 : i + 1];
         // LDS u16 offset of cell (i, 0), mod 2^32
         const uint32_t bo = (uint32_t)(rare_row_base(jb, it, i) - it.a0) + 8u * it.lb;
         for (uint32_t eb = e0; eb < e1; eb += RNT) {
@@ -1271,15 +1295,27 @@ __device__ __forceinline__ void rare_ranges_flat(const RareJob& jb, const M* __r
                 W += s;
             }
             const uint32_t ws = wpre + incl - nw;
+            uint32_t bor = bo;  // the range's cell (row, 0)
             if (tid < nb) {
                 wstart[tid] = ws;
                 wdelta[tid] = wa - ws;
+                if constexpr (MR) {
+                    // the range's row: rs[lo] <= eb + tid < rs[hi] throughout
+                    const uint32_t e = eb + (uint32_t)tid;
+                    int lo = 0, hi = it.i1 - it.i0;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (sh.rs[mid] <= e) lo = mid; else hi = mid;
+                    }
+                    bor = bo + (uint32_t)lo * pitch;
+                    sh.wbo[tid] = bor;
+                }
             }
             if (tid < nb && end > b) {
                 const bool ef = (b % PER) != 0 || nw == 0;
                 const bool el = (end % PER) != 0 && lw != fw;
-                edge(xf, fw, ef ? b : 0u, ef ? min(end, (fw + 1) * PER) : 0u, bo);
-                edge(xl, lw, el ? lw * PER : 0u, el ? end : 0u, bo);
+                edge(xf, fw, ef ? b : 0u, ef ? min(end, (fw + 1) * PER) : 0u, bor);
+                edge(xl, lw, el ? lw * PER : 0u, el ? end : 0u, bor);
             }
             asm volatile("" : : "v"(xf[0]), "v"(xl[0]));
             __syncthreads();
@@ -1353,6 +1389,23 @@ __device__ __forceinline__ void rare_ranges_flat(const RareJob& jb, const M* __r
                     else if constexpr (Q == 2) return buf.s89ab;
                     else return buf.scdef;
                 };
+                // MR: the cell offset of each set's words (their ranges' rows)
+                uint32_t bq0 = bo, bq1 = bo, bq2 = bo, bq3 = bo;
+                auto setb = [&](auto q, uint32_t x) {
+                    constexpr int Q = decltype(q)::value;
+                    if constexpr (Q == 0) bq0 = x;
+                    else if constexpr (Q == 1) bq1 = x;
+                    else if constexpr (Q == 2) bq2 = x;
+                    else bq3 = x;
+                };
+                auto getb = [&](auto q) -> uint32_t {
+                    constexpr int Q = decltype(q)::value;
+                    if constexpr (!MR) return bo;
+                    else if constexpr (Q == 0) return bq0;
+                    else if constexpr (Q == 1) return bq1;
+                    else if constexpr (Q == 2) return bq2;
+                    else return bq3;
+                };
                 int rl;  // lane's range in the chunk at k0
                 {
                     const int rr = r + 1 + lane;
@@ -1367,13 +1420,15 @@ __device__ __forceinline__ void rare_ranges_flat(const RareJob& jb, const M* __r
                 // set L and look up the next chunk's ranges
                 auto step = [&](auto l, auto a, auto apply) {
                     const uint32_t d = wdelta[rl];
+                    const uint32_t bl = MR ? sh.wbo[rl] : 0u;
                     const int rr = r + 1 + lane;
                     const uint32_t s0 = starts_at(rr);
                     __builtin_amdgcn_sched_barrier(0);
-                    if constexpr (decltype(apply)::value) whole(get(a), bo);
+                    if constexpr (decltype(apply)::value) whole(get(a), getb(a));
                     __builtin_amdgcn_sched_barrier(0);
                     const uint32_t k = k0 + lane;
                     set(l, words[k < kend ? k + d : 0u]);
+                    if constexpr (MR) setb(l, bl);
                     k0 += 64;
                     advance(k0, rr < nb ? s0 : ~0u, rl);
                 };
@@ -1395,7 +1450,8 @@ __device__ __forceinline__ void rare_ranges_flat(const RareJob& jb, const M* __r
                 if (nfull * 64 < kend - kbeg) {
                     const uint32_t q = nfull - c;
                     const v4i t = q == 0 ? get(I0{}) : q == 1 ? get(I1{}) : q == 2 ? get(I2{}) : get(I3{});
-                    if (kbeg + nfull * 64 + lane < kend) whole(t, bo);
+                    const uint32_t tb = q == 0 ? getb(I0{}) : q == 1 ? getb(I1{}) : q == 2 ? getb(I2{}) : getb(I3{});
+                    if (kbeg + nfull * 64 + lane < kend) whole(t, tb);
                 }
                 // the last loads may not be used: keep every set live so that
                 // no load can land in a register that is already reused
@@ -1469,13 +1525,15 @@ __device__ __forceinline__ void rare_items(const RareJob& jb, const M* __restric
             nxt.lb = cur.lb ? 0 : RHALF;
             rare_stage_dma(jb, nxt, sh);
         }
-        // Ascending ranges: one-row items take the flat walk, items of several
-        // short rows the subgroup walk, whose fixed cost per row is lower (the
-        // flat walk on those made a batch of 64 sets of 500 to 20,000 rows
-        // 16% slower; on the 63,005-row long-tail set, one-row items only,
-        // it takes 4.8 ms against 5.2 ms).
+        // Ascending ranges: one-row items take the flat walk (on the 63,005-row
+        // long-tail set, one-row items only, 4.8 ms against 5.2 ms for the
+        // subgroup walk); items of several rows take it too when the ranges
+        // are long (jb.mflat), else the subgroup walk, whose cost per short
+        // range is lower (the flat walk made the batch of 64 sets of 500 to
+        // 20,000 rows 18% slower).
         if constexpr (MASK) rare_ranges<M, true>(jb, gmem, cur, sh);
-        else if (cur.i1 - cur.i0 == 1) rare_ranges_flat<M>(jb, gmem, cur, sh);
+        else if (cur.i1 - cur.i0 == 1) rare_ranges_flat<M, false>(jb, gmem, cur, sh);
+        else if (jb.mflat && cur.i1 - cur.i0 <= RFG) rare_ranges_flat<M, true>(jb, gmem, cur, sh);
         else rare_ranges<M, false>(jb, gmem, cur, sh);
         if (more && !dma) rare_stage_issue(jb, nxt, v, SEQ);  // nxt.lb == 0
         rare_write_back(jb, cur, sh);

@@ -444,6 +444,48 @@ def test_rare_flat_walk_one_row_items(cgh, gpu, workloads):
             assert np.array_equal(got, np.minimum(ref[i], cap) if cap else ref[i]), (tau, kernel, cap, i)
 
 
+@pytest.mark.parametrize("n", [20000, 5000])
+def test_rare_flat_walk_multi_row_items(cgh, gpu, workloads, n):
+    """Rows short enough that several share an LDS chunk (3 rows of 20,000,
+    13 rows of 5,000 per work item).  With long member ranges (a mean of 160
+    partners or more per range) such items take the flat walk over all their
+    rows' entries at once: batches cross row boundaries, each range's row is
+    searched from the rows' starts and carried with its member words.  With
+    short ranges (tau 256, a mean near 40) they take the subgroup walk.  The
+    whole upper triangle, chained into tc_hamming.cu:241's checksum, equals
+    the oracle's either way; rows at the stripe and item edges equal the
+    oracle's rows, also under a cap."""
+    L, S = 1142, 4096
+    X = cgh.dense_ranks(workloads.long_tail(n, L, seed=37))
+    print(f"multi-row flat walk n={n}: oracle checksum", flush=True)
+    want = oracle.checksum_upper(X)
+    rows = [0, 1, 2, 3, 4095, 4096, n // 2, n - 2]
+    ref = {i: oracle.brute_rows(X, i, i + 1)[0, i + 1:] for i in rows}
+    out = gpu.empty(cgh.upper_elems(n, S), dtype=gpu.int16, device="cuda")
+    st = gpu.cuda.current_stream()
+    for tau, flat, cap in ((256, False, 0), (2048, True, 0), (10**6, True, 0), (2048, True, 250)):
+        enc = cgh.Encoded(X, tau, 256)
+        mean = int(enc.enc.n_rare) / max(1, int(enc.enc.n_rentries))
+        assert (mean >= 160) == flat, (tau, mean)  # which walk the items take
+        print(f"multi-row flat walk n={n}: tau={tau} mean={mean:.0f} cap={cap}", flush=True)
+        ds = cgh.Dataset(enc)
+        if cap:
+            ds.set_cap(cap)
+        out.fill_(0x6b6b)
+        ds.upper(S, out.data_ptr(), st.cuda_stream, 2)
+        gpu.cuda.synchronize()
+        ds.close()
+        flat_out = out.cpu().numpy().view(np.uint16)
+        if not cap:
+            assert upper_checksum(cgh, flat_out, n, S) == want, (n, tau)
+        for i in rows:
+            s = i // S
+            r0 = s * S
+            base = cgh.stripe_offset(n, S, s) + (i - r0) * (n - r0)
+            got = flat_out[base + i + 1 - r0: base + n - r0]
+            assert np.array_equal(got, np.minimum(ref[i], cap) if cap else ref[i]), (n, tau, cap, i)
+
+
 @pytest.mark.parametrize("cap", [4464, 65535, 65536, 70000])
 def test_cap_at_and_beyond_u16(cgh, gpu, cap):
     """-X at and past the u16 range on rows that the rare pass finishes (tau
