@@ -200,9 +200,9 @@ def headline(args, cgh, wl, torch):
                          f"({members} members, {entries} ranges); the {R} member reads (2 B each) hit the caches",
                  "ms": rare_ms, "frac_of_measured_copy": rare_gbs / memory["copy_gbs"],
                  "rare_pairs_per_s": R / (rare_ms * 1e-3),
-                 "limiter": "per-item chain of memory round trips (cell staging, range entries and edge "
-                            "words, then the walk's first chunk), which every wave of the CU waits for at "
-                            "the item barriers; ~11 vector instructions per rare pair; see "
+                 "limiter": "the walk (about 3.4 ms with staging and write-back switched off; ~11 vector "
+                            "instructions per rare pair) plus the staging and write-back of items too wide "
+                            "to be copied into LDS during the walk before them; see "
                             "profiles/sub_rare_rows_counters_*.md"},
     }
     if traffic["rare"]:  # L2 <-> fabric bytes the counters saw, against the cache-resident read rate
