@@ -579,15 +579,27 @@ extern "C" long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int 
             memset(O + (size_t)r * c->kw1, 0, c->kw1 * 4);
             memset(P + (size_t)r * c->kw2, 0, c->kw2 * 4);
         }
+        // each call's home slot is hashed and its line prefetched one locus
+        // ahead, so the block's probes of a locus overlap their cache misses
+        size_t home[2][16];
+        auto prefetch = [&](int k, size_t* hk) {
+            const cgh_codec::Slot* t = c->slots.data() + c->off[k];
+            for (int r = r0; r < r1; r++) {
+                hk[r - r0] = codec_hash(calls[(size_t)r * L + k], c->shift[k]);
+                __builtin_prefetch(t + hk[r - r0]);
+            }
+        };
+        prefetch(0, home[0]);
         for (int k = 0; k < L; k++) {
             const cgh_codec::Slot* t = c->slots.data() + c->off[k];
             const size_t mask = c->off[k + 1] - c->off[k] - 1;
-            const int sh = c->shift[k];
+            const size_t* hk = home[k & 1];
+            if (k + 1 < L) prefetch(k + 1, home[(k + 1) & 1]);
             for (int r = r0; r < r1; r++) {
                 const uint64_t v = calls[(size_t)r * L + k];
                 if (!v) continue;
                 P[(size_t)r * c->kw2 + (k >> 5)] |= 1u << (k & 31);
-                size_t h = codec_hash(v, sh);
+                size_t h = hk[r - r0];
                 while (t[h].id && t[h].id != v) h = (h + 1) & mask;
                 if (!t[h].id) continue;  // an id the database never saw matches no row
                 const uint32_t x = t[h].x, y = t[h].y;
