@@ -31,7 +31,8 @@ EXPORTS = (
     "cgh_dataset_upper", "cgh_dataset_phase", "cgh_dataset_stream", "cgh_dataset_stream_graph",
     "cgh_batch_create", "cgh_batch_out_elems", "cgh_batch_out_offset", "cgh_batch_tile", "cgh_batch_run",
     "cgh_batch_free",
-    "cgh_codec_create", "cgh_codec_info", "cgh_codec_members", "cgh_codec_encode", "cgh_codec_free",
+    "cgh_codec_create", "cgh_codec_info", "cgh_codec_members", "cgh_codec_encode", "cgh_codec_encode_rows",
+    "cgh_codec_free",
     "cgh_db_create", "cgh_db_info", "cgh_db_free",
     "cgh_server_create", "cgh_server_slots", "cgh_server_submit", "cgh_server_wait", "cgh_server_free",
     "cgh_measure_peaks", "cgh_measure_memory", "cgh_dataset_tile", "cgh_batch_run_noalloc",
@@ -111,6 +112,7 @@ def load():
         "cgh_codec_info": (None, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)] + [C.POINTER(S)] * 4),
         "cgh_codec_members": (C.POINTER(C.c_uint32), [P]),
         "cgh_codec_encode": (C.c_long, [P, P, I, P, P, P, P, S, C.c_char_p, I]),
+        "cgh_codec_encode_rows": (C.c_long, [P, P, I, P, P, P, P, S, C.c_char_p, I]),
         "cgh_codec_free": (None, [P]),
         "cgh_db_create": (I, [I, P, I, I, I, C.POINTER(P), C.c_char_p, I]),
         "cgh_db_info": (I, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(S)]),
@@ -472,6 +474,24 @@ class QueryCodec:
         err = _err()
         m = load().cgh_codec_encode(self.handle, _ptr(calls), q, _ptr(O), _ptr(P), _ptr(rs), _ptr(re), re.size,
                                     err, 512)
+        if m < 0:
+            raise CghError(err.value.decode(errors="replace"))
+        return O, P, rs, re[:m].copy()
+
+    def encode_rows(self, parts):
+        """encode() of the queries of several arrays (each q_i x L) in one
+        call through cgh_codec_encode_rows, as if they were stacked."""
+        parts = [np.ascontiguousarray(a, dtype=np.uint64) for a in parts]
+        rows = [a[i].ctypes.data for a in parts for i in range(a.shape[0])]
+        q = len(rows)
+        ptrs = (C.c_void_p * max(q, 1))(*rows)
+        O = np.full((q, self.kw1), 0xFFFFFFFF, dtype=np.uint32)
+        P = np.full((q, self.kw2), 0xFFFFFFFF, dtype=np.uint32)
+        rs = np.zeros(q + 1, dtype=np.uint32)
+        re = np.zeros(max(q * self.L, 1), dtype=np.uint64)
+        err = _err()
+        m = load().cgh_codec_encode_rows(self.handle, C.cast(ptrs, C.c_void_p), q, _ptr(O), _ptr(P), _ptr(rs),
+                                         _ptr(re), re.size, err, 512)
         if m < 0:
             raise CghError(err.value.decode(errors="replace"))
         return O, P, rs, re[:m].copy()

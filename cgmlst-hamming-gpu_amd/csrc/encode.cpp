@@ -556,8 +556,11 @@ extern "C" void cgh_codec_info(const cgh_codec* c, int* n, int* L, int* tau, siz
 
 extern "C" const uint32_t* cgh_codec_members(const cgh_codec* c) { return c->members.data(); }
 
-extern "C" long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int q, uint32_t* O, uint32_t* P,
-                                 uint32_t* rstart, uint64_t* rentry, size_t cap, char* err, int errlen) {
+namespace {
+// Query row r's L calls are row(r).
+template <typename Row>
+long codec_encode(const cgh_codec* c, Row row, int q, uint32_t* O, uint32_t* P, uint32_t* rstart, uint64_t* rentry,
+                  size_t cap, char* err, int errlen) {
     const int L = c->L;
     if (q < 0 || cap < (size_t)q * (size_t)L) {
         set_err(err, errlen, "cgh_codec_encode: %d queries need room for %zu rare entries (have %zu)", q,
@@ -585,7 +588,7 @@ extern "C" long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int 
         auto prefetch = [&](int k, size_t* hk) {
             const cgh_codec::Slot* t = c->slots.data() + c->off[k];
             for (int r = r0; r < r1; r++) {
-                hk[r - r0] = codec_hash(calls[(size_t)r * L + k], c->shift[k]);
+                hk[r - r0] = codec_hash(row(r)[k], c->shift[k]);
                 __builtin_prefetch(t + hk[r - r0]);
             }
         };
@@ -596,7 +599,7 @@ extern "C" long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int 
             const size_t* hk = home[k & 1];
             if (k + 1 < L) prefetch(k + 1, home[(k + 1) & 1]);
             for (int r = r0; r < r1; r++) {
-                const uint64_t v = calls[(size_t)r * L + k];
+                const uint64_t v = row(r)[k];
                 if (!v) continue;
                 P[(size_t)r * c->kw2 + (k >> 5)] |= 1u << (k & 31);
                 size_t h = hk[r - r0];
@@ -626,4 +629,18 @@ extern "C" long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int 
         rstart[r + 1] = (uint32_t)tot;
     }
     return (long)tot;
+}
+}  // namespace
+
+extern "C" long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int q, uint32_t* O, uint32_t* P,
+                                 uint32_t* rstart, uint64_t* rentry, size_t cap, char* err, int errlen) {
+    const size_t L = (size_t)c->L;
+    return codec_encode(c, [calls, L](int r) { return calls + (size_t)r * L; }, q, O, P, rstart, rentry, cap, err,
+                        errlen);
+}
+
+extern "C" long cgh_codec_encode_rows(const cgh_codec* c, const uint64_t* const* rows, int q, uint32_t* O,
+                                      uint32_t* P, uint32_t* rstart, uint64_t* rentry, size_t cap, char* err,
+                                      int errlen) {
+    return codec_encode(c, [rows](int r) { return rows[r]; }, q, O, P, rstart, rentry, cap, err, errlen);
 }

@@ -3067,6 +3067,7 @@ struct cgh_server {
         uint16_t *hdist = NULL, *htopd = NULL;
         uint32_t* htopi = NULL;
         size_t re_cap = 0;
+        std::vector<const uint64_t*> rows;  // the batch's query rows, request by request
     };
     cgh_db* db = NULL;
     int max_batch = 0, nslots = 0, kernel = 0;
@@ -3099,17 +3100,15 @@ size_t batch_bytes(const cgh_db* db, int q) {
 hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
     cgh_db* db = sv->db;
     const int q = sl.q, q_pad = (int)up((size_t)q, 256);
-    // queries in the database's columns, request by request into the
-    // slot's pinned buffers; rows q..q_pad-1 stay zero
-    size_t row = 0, ne = 0;
-    for (auto* r : sl.reqs) {
-        const long m = cgh_codec_encode(db->codec, r->calls.data(), r->q, sl.hO + row * db->kw1,
-                                        sl.hP + row * db->kw2, sl.hrs + row, sl.hre + ne, sl.re_cap - ne, NULL, 0);
-        if (m < 0) return hipErrorInvalidValue;
-        for (int k = 0; k <= r->q; k++) sl.hrs[row + k] += (uint32_t)ne;
-        row += (size_t)r->q;
-        ne += (size_t)m;
-    }
+    // the batch's queries in the database's columns, all requests in one
+    // call (one parallel region over every query) into the slot's pinned
+    // buffers; rows q..q_pad-1 stay zero
+    sl.rows.clear();
+    for (auto* r : sl.reqs)
+        for (int k = 0; k < r->q; k++) sl.rows.push_back(r->calls.data() + (size_t)k * db->L);
+    const long m = cgh_codec_encode_rows(db->codec, sl.rows.data(), q, sl.hO, sl.hP, sl.hrs, sl.hre, sl.re_cap, NULL, 0);
+    if (m < 0) return hipErrorInvalidValue;
+    const size_t ne = (size_t)m;
     memset(sl.hO + (size_t)q * db->kw1, 0, (size_t)(q_pad - q) * db->kw1 * 4);
     memset(sl.hP + (size_t)q * db->kw2, 0, (size_t)(q_pad - q) * db->kw2 * 4);
     hipError_t e;
@@ -3143,7 +3142,7 @@ hipError_t launch_batch(cgh_server* sv, cgh_server::Slot& sl) {
                        tchunks, sl.dpart);
     hipLaunchKernelGGL(top_k_final, dim3((unsigned)q), dim3(256), 0, sl.st, sl.dpart, tchunks, sl.dtopd, sl.dtopi);
     if ((e = hipGetLastError())) return e;
-    row = 0;  // distances only of the requests that asked for them
+    size_t row = 0;  // distances only of the requests that asked for them
     for (auto* r : sl.reqs) {
         const size_t off = row * (size_t)db->n, cnt = (size_t)r->q * db->n;
         if (r->want_dist && (e = hipMemcpyAsync(sl.hdist + off, sl.ddist + off, cnt * 2, hipMemcpyDeviceToHost, sl.st)))

@@ -344,6 +344,11 @@ const uint32_t* cgh_codec_members(const cgh_codec* c);
  * H = |P_r & P_j| - |O_r & O_j| - #{ranges of r holding j}. */
 long cgh_codec_encode(const cgh_codec* c, const uint64_t* calls, int q, uint32_t* O, uint32_t* P, uint32_t* rstart,
                       uint64_t* rentry, size_t cap, char* err, int errlen);
+/* cgh_codec_encode with query r's L calls at rows[r] (queries from several
+ * buffers encoded in one call, as the server does for a batch of requests).
+ * The library's own; no counterpart in the reference. */
+long cgh_codec_encode_rows(const cgh_codec* c, const uint64_t* const* rows, int q, uint32_t* O, uint32_t* P,
+                           uint32_t* rstart, uint64_t* rentry, size_t cap, char* err, int errlen);
 void cgh_codec_free(cgh_codec* c);
 
 /* Database of n profiles x L loci (u64 allele ids, 0 = missing) resident on

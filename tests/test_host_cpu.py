@@ -173,6 +173,20 @@ def test_query_codec_identity(cgh, workloads, tau):
     codec.close()
 
 
+def test_query_codec_rows_match_stacked(cgh, workloads):
+    """cgh_codec_encode_rows over queries held in separate buffers (the
+    server's batch of requests) gives exactly cgh_codec_encode's output for
+    the same queries stacked in one array: bits, row starts and entries."""
+    db_calls, q_calls = workloads.serving_sets(800, 120, 61, seed=5)
+    codec = cgh.QueryCodec(db_calls, 16)
+    want = codec.encode(q_calls)
+    parts = [q_calls[:1], q_calls[1:20], q_calls[20:21], q_calls[21:]]
+    got = codec.encode_rows(parts)
+    for w, g in zip(want, got):
+        assert np.array_equal(w, g)
+    codec.close()
+
+
 def test_query_codec_rejects_short_buffer(cgh, workloads):
     db_calls, q_calls = workloads.serving_sets(200, 40, 5, seed=3)
     codec = cgh.QueryCodec(db_calls, 8)
